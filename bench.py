@@ -1,0 +1,172 @@
+"""Benchmark: SL-BDF2 steps/s of one channel at 256^3, triquintic, float64.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--order 6] [--n 256]
+
+Workload (the single-GPU configuration of the headline metric): 256x256x256
+channel, Lx = 4 pi, Ly = 2 pi, Lz = 2, tanh factor 1.5, Re_b = 2792.8,
+interp_order 6, seeded Poiseuille + solenoidal k^-5/3 perturbation over
+1 <= |k| <= 32 with u'rms = 0.15, dt+ = 0.2 (Re_tau = 180).  A "step" is one
+full SL-BDF2 step (predictor, z-solves, FFT projection, bulk shift); the W
+warm-up steps include the BDF1 bootstrap.  Inputs are resident in HBM.
+
+Prints ONE JSON line.  Also reported:
+  roofline      the dominant kernel (the near-foot BDF2 pass of u) timed with
+                hipEvents on its launch stream over the timed region;
+                algorithmic bytes = reads of the six velocity arrays + the far
+                value + the write, 64 B per arrival point;
+  gather_GBps   the isolated order-P gather of u (slc_gather) at 40 B/point
+                (3x8 B feet + 8 B field + 8 B out), feet from the 2dt foot;
+  cpu_baseline  the CPU oracle (oracle/: numpy + OpenMP C) timed on this
+                host for ONE BDF2 step of the same 256^3 state.
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+PEAK_HBM_GBPS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--order", type=int, default=6)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--nz", type=int, default=None)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build runs the "
+              "channel on a single MI355X and has no multi-GPU path.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from slchannel_amd import SLChannelFlow
+
+    n, nz = args.n, args.nz or args.n
+    Re_b, Re_tau, dtp = 2792.8, 180.0, 0.2
+    cfg = {
+        "grid": {"nx": n, "ny": n, "nz": nz},
+        "domain": {"Lx": 4 * math.pi, "Ly": 2 * math.pi, "Lz": 2.0},
+        "flow": {"Re": Re_b, "U_bulk": 1.0, "gamma": 1.5},
+        "sl": {"interp_order": args.order},
+    }
+    flow = SLChannelFlow(config=cfg, device="cuda:0")
+    flow.initialize(seed=2000, urms=0.15, kmin=1.0, kmax=32.0, spectrum="k-5/3")
+    nu = flow.nu
+    u_tau = Re_tau * nu  # delta = 1
+    dt = dtp * nu / (u_tau * u_tau)
+
+    for _ in range(args.warmup):
+        flow.step_sl_bdf2(dt)
+    torch.cuda.synchronize()
+
+    plan = flow.plan
+    plan.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ub, fo = flow.step_sl_bdf2(dt)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = plan.profile_report()
+    plan.profile(False)
+    ms_step = (t1 - t0) * 1e3 / args.steps
+
+    N_u = n * n * nz
+    lab = "bdf2_near_u"
+    cnt, tot = prof[lab]
+    avg_ms = tot / cnt
+    bytes_near = 64.0 * N_u
+    achieved = bytes_near / (avg_ms * 1e-3) / 1e9
+
+    # isolated gather of u at the 2dt foot of the current state
+    xd, yd, zd = (torch.empty(N_u, dtype=torch.float64, device="cuda") for _ in range(3))
+    mids = [(2.0 * a - b).contiguous() for a, b in ((flow.u, flow.u_nm1), (flow.v, flow.v_nm1),
+                                                    (flow.w, flow.w_nm1))]
+    plan.departure("u", *mids, 2.0 * dt, xd, yd, zd)
+    out = torch.empty(N_u, dtype=torch.float64, device="cuda")
+    plan.gather("u", flow.u_nm1, xd, yd, zd, out)
+    torch.cuda.synchronize()
+    plan.profile(True)
+    for _ in range(5):
+        plan.gather("u", flow.u_nm1, xd, yd, zd, out)
+    gp = plan.profile_report()
+    plan.profile(False)
+    g_cnt, g_tot = gp["gather_u"]
+    gather_gbps = 40.0 * N_u / (g_tot / g_cnt * 1e-3) / 1e9
+
+    # correctness guards on the measured run (no host sync inside the loop)
+    div = flow.divergence().abs().max().item()
+    ubv = ub.item()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(flow, dt)
+
+    kernels = {k: {"launches": c, "avg_ms": round(t / c, 4)} for k, (c, t) in sorted(prof.items())}
+    line = {
+        "metric": "steps/s (SL-BDF2 step, 256^3 triquintic fp64)",
+        "value": round(args.steps / (t1 - t0), 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded solenoidal k^-5/3 perturbation on Poiseuille, projected)",
+        "config": {"workload": f"SL-BDF2 channel {n}x{n}x{nz} order {args.order} "
+                               f"Re_b {Re_b} dt+ {dtp}",
+                   "nx": n, "ny": n, "nz": nz, "interp_order": args.order, "dt": dt},
+        "roofline": {"bound": "hbm", "kernel": "bdf2_near_kernel<u,P>",
+                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
+                     "bytes_per_launch": bytes_near, "avg_ms": round(avg_ms, 4)},
+        "gather_GBps": round(gather_gbps, 1),
+        "gather_frac_hbm": round(gather_gbps / PEAK_HBM_GBPS, 4),
+        "max_div": div,
+        "u_bulk": ubv,
+        "kernels_ms": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(flow, dt):
+    """One BDF2 step of the CPU oracle on the GPU's current 256^3 state."""
+    import numpy as np
+
+    from oracle import oracle as orc
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    co = orc.ChannelOracle(flow.nx, flow.ny, flow.nz, flow.Lx, flow.Ly, flow.Lz, flow.gamma,
+                           1.0 / flow.nu, order=flow.sl_order, z_f=flow.z_f.numpy(),
+                           z_c=flow.z_c.numpy(), tanh_g=flow.plan.tanh_g)
+    co.set_fields(*(t.cpu().numpy() for t in (flow.u, flow.v, flow.w, flow.p)))
+    co.set_history(*(t.cpu().numpy() for t in (flow.u_nm1, flow.v_nm1, flow.w_nm1)), dt)
+    t0 = time.perf_counter()
+    co.step_sl_bdf2(dt)
+    sec = time.perf_counter() - t0
+    del np
+    return {"value": round(1.0 / sec, 5), "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"one BDF2 step of the CPU oracle at {flow.nx}x{flow.ny}x{flow.nz} "
+                      f"(numpy + OpenMP C, {threads} threads; {sec:.1f} s)"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,187 @@
+/*
+ * slc.h -- C ABI of libslc_hip.so, the MI355X (gfx950) performance layer for
+ * slChannel's semi-Lagrangian BDF2 step and the pressure projection around it.
+ *
+ * Conventions
+ *   - Every entry point returns an int status: 0 (SLC_OK) or a negative
+ *     SLC_E* code; nothing throws across the ABI.  slc_last_error() gives a
+ *     human-readable message for the calling thread.
+ *   - Array arguments are DEVICE pointers to float64 data owned by the caller
+ *     (borrowed, never freed).  Fields are the reference's ghost-shaped
+ *     staggered arrays, C order with z fastest:
+ *        u (nx+1, ny+2, nz+2)   v (nx+2, ny+1, nz+2)
+ *        w (nx+2, ny+2, nz+1)   p (nx+2, ny+2, nz+2)
+ *   - `stream` is a hipStream_t (NULL = the legacy default stream).  All work
+ *     is enqueued on it; no entry point synchronises the host, so results
+ *     (including the clamp counter and the bulk scalars) stay on the device.
+ *   - Components are numbered 0 = u, 1 = v, 2 = w.
+ *
+ * Each function names the reference interface (gmcavallazzi/SLchannel,
+ * file:line) it replaces.
+ */
+#ifndef SLC_H
+#define SLC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SLC_OK 0
+#define SLC_EINVAL (-1)   /* bad argument / unsupported option */
+#define SLC_EHIP (-2)     /* HIP runtime error */
+#define SLC_EFFT (-3)     /* rocFFT error */
+#define SLC_ENOMEM (-4)   /* device allocation failed */
+
+typedef struct slc_plan slc_plan;
+
+/* Grid and scheme description.  Mirrors the arguments of
+ * SLAdvector.__init__ (slchannel/semilag.py:133-153) plus the solver scalars
+ * the step needs (slchannel/solver.py:195-255, 311-321). */
+typedef struct {
+    int nx, ny, nz;        /* interior cells; x, y periodic, z wall-bounded */
+    double Lx, Ly, Lz;     /* dx = Lx/nx, dy = Ly/ny (solver.py:315-316) */
+    double gamma;          /* tanh stretching factor */
+    double tanh_g;         /* tanh(gamma) as the host computed it (semilag.py:240-242) */
+    int stretching;        /* 0 = 'symmetric', 1 = 'bottom' (semilag.py:210-213) */
+    int order;             /* 4 | 6 (semilag.py:204-205) */
+    int traj_order;        /* 2 (trilinear trajectory sampling; semilag.py:206-209) */
+    int n_traj_iters;      /* midpoint fixed-point iterations (semilag.py:543) */
+    int top_wall;          /* 0 = 'dirichlet' (no-slip), 1 = 'neumann' (free-slip) */
+    double nu;             /* 1/Re (solver.py:209) */
+    const double* z_f;     /* HOST pointer, nz+1 faces   (utils.py:21-61) */
+    const double* z_c;     /* HOST pointer, nz+2 centres (utils.py:21-61) */
+} slc_grid_desc;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int slc_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* slc_last_error(void);
+
+/* Build a plan on HIP device `device`: uploads the z tables, derives the
+ * Lagrange denominator tables (semilag.py:102-127), the Poisson tables
+ * (projection.py:24-131, kept as 1-D factors, not (nx, ny/2+1, nz) arrays),
+ * creates the rocFFT plans and the work buffers.
+ * Replaces SLAdvector.__init__ (semilag.py:133-381), TritonSL.__init__
+ * (semilag_triton.py:352-382) and initialize_fft_solver (projection.py:24-131). */
+int slc_plan_create(const slc_grid_desc* desc, int device, slc_plan** out);
+
+/* Free a plan and its device buffers. */
+int slc_plan_destroy(slc_plan* plan);
+
+/* Device bytes held by the plan (tables + work buffers). */
+int slc_plan_workspace_bytes(const slc_plan* plan, int64_t* out);
+
+/* Ghost refresh of the three velocity fields (any may be NULL).
+ * Replaces apply_bc_all (slchannel/solver.py:70-107). */
+int slc_apply_bc(slc_plan* plan, double* u, double* v, double* w, void* stream);
+
+/* Departure feet of every arrival point of component `comp` by the iterated
+ * midpoint rule with trilinear sampling of the trajectory velocity
+ * (um, vm, wm: ghost-shaped).  Outputs are flat arrays of nx*ny*nzi doubles
+ * (nzi = nz for u, v and nz-1 for w, z fastest); x, y unwrapped, z NOT
+ * clamped.  Replaces SLAdvector.departure_coords (semilag.py:534-552) /
+ * TritonSL.departure (semilag_triton.py:384-420). */
+int slc_departure(slc_plan* plan, int comp, const double* um, const double* vm,
+                  const double* wm, double dt, double* xd, double* yd, double* zd,
+                  void* stream);
+
+/* Clamp the feet into [z_lo, z_hi] (adding the number clamped to *nclamp, a
+ * device int64, if non-NULL), build the order-P stencils and interpolate
+ * `field`.  layout 0: `field` is the component's ghost-shaped array;
+ * layout 1: a packed node buffer (nx, ny, NZ) as built by SLAdvector._fill
+ * (NZ = nz+2 for u, v; nz+1 for w).  out: flat nx*ny*nzi.
+ * Replaces _build_iw_impl + _apply_iw (semilag.py:471-510) /
+ * TritonSL.gather (semilag_triton.py:422-455). */
+int slc_gather(slc_plan* plan, int comp, const double* field, int layout, const double* xd,
+               const double* yd, const double* zd, double* out, int64_t* nclamp,
+               void* stream);
+
+/* Stencil base indices of given feet after the clamp: ix/iy = node index of
+ * the first stencil point (offset -(P/2-1)) after the periodic wrap, k0 = the
+ * clamped z base (int32 arrays of nx*ny*nzi).  Adds the clamp count to
+ * *nclamp if non-NULL.  Replaces the index half of
+ * SLAdvector.compute_departure -> IndexWeights (semilag.py:471-500, 573-578). */
+int slc_stencil(slc_plan* plan, int comp, const double* xd, const double* yd, const double* zd,
+                int32_t* ix, int32_t* iy, int32_t* k0, int64_t* nclamp, void* stream);
+
+/* Full semi-Lagrangian advection of (u, v, w) along (um, vm, wm) over dt,
+ * writing the interiors of the ghost-shaped outputs (ghosts and the w wall
+ * faces untouched) and adding the clamp count to *nclamp (device int64,
+ * may be NULL).  Replaces SLAdvector.advect (semilag.py:584-671). */
+int slc_advect(slc_plan* plan, const double* u, const double* v, const double* w,
+               const double* um, const double* vm, const double* wm, double dt,
+               double* ustar, double* vstar, double* wstar, int64_t* nclamp, void* stream);
+
+/* BDF1 bootstrap predictor: ustar = advect(V^n; U* = V^n, dt) + dt * nu *
+ * L_xy V^n on the interiors.  Replaces solver.py:737-764. */
+int slc_bdf1_predict(slc_plan* plan, const double* u, const double* v, const double* w,
+                     double dt, double* ustar, double* vstar, double* wstar, int64_t* nclamp,
+                     void* stream);
+
+/* BDF2 predictor with the frozen U* = 2V^n - V^{n-1}: two independent feet
+ * (dt, 2dt), V^n interpolated at the near foot and V^{n-1} at the far foot,
+ * u_hat = 4/3 near - 1/3 far + (2dt/3)(2 R^n - R^{n-1}) with R = nu L_xy V.
+ * Clamp counts of both feet are added to *nclamp.
+ * Replaces solver.py:846-903 ('noninc' pressure, 'extrap' xy RHS). */
+int slc_bdf2_predict(slc_plan* plan, const double* u, const double* v, const double* w,
+                     const double* unm1, const double* vnm1, const double* wnm1, double dt,
+                     double* ustar, double* vstar, double* wstar, int64_t* nclamp,
+                     void* stream);
+
+/* In-place theta-method implicit z-diffusion of one component's interior
+ * (ghosts untouched).  Replaces solve_implicit_diffusion_{u,v,w}
+ * (slchannel/operators.py:382-630). */
+int slc_zsolve(slc_plan* plan, int comp, double* field, double dt, double theta, void* stream);
+
+/* Divergence of the velocity into a (nx, ny, nz) z-contiguous array.
+ * Replaces compute_divergence (slchannel/utils.py:166-180). */
+int slc_divergence(slc_plan* plan, const double* u, const double* v, const double* w,
+                   double* div, void* stream);
+
+/* Solve the pressure Poisson equation for rhs = div / rhs_scale (div a
+ * (nx, ny, nz) z-contiguous array) into the ghost-shaped p.
+ * Replaces solve_poisson_fft (slchannel/projection.py:133-197). */
+int slc_poisson(slc_plan* plan, const double* div, double rhs_scale, double* p, void* stream);
+
+/* u -= dt grad p on the interiors.  Replaces project_velocity
+ * (slchannel/projection.py:250-272). */
+int slc_project(slc_plan* plan, double* u, double* v, double* w, const double* p, double dt,
+                void* stream);
+
+/* The projection block of one step, fused: div(u)/rhs_scale -> rocFFT ->
+ * per-mode Thomas -> inverse rocFFT -> u -= dt_proj grad p -> ghosts of
+ * u, v, w and p.  p may be NULL (pressure not materialised).
+ * Replaces solver.py:983-998 + 1015 (and 797-801 + 809 in the bootstrap). */
+int slc_project_step(slc_plan* plan, double* u, double* v, double* w, double rhs_scale,
+                     double dt_proj, double* p, void* stream);
+
+/* Bulk velocity sum(u dx dy dz_f)/V of u's interior into out[3] (out: 4
+ * device doubles of scratch).  Replaces compute_bulk_velocity
+ * (slchannel/utils.py:147-163). */
+int slc_bulk_velocity(slc_plan* plan, const double* u, double* out, void* stream);
+
+/* Exact bulk-flux constraint: u += U_bulk - ub(u) on the interior, ghosts of
+ * u refreshed.  out (4 device doubles): [0] correction, [1] forcing =
+ * correction/dt, [2] bulk velocity after the shift, [3] before it.
+ * Replaces SLChannelFlow._apply_bulk_forcing (solver.py:694-723). */
+int slc_bulk_force(slc_plan* plan, double* u, double* v, double* w, double U_bulk, double dt,
+                   double* out, void* stream);
+
+/* Opt-in launch timing (no counterpart in the reference, whose benchmarks
+ * time whole steps with perf_counter + cuda.synchronize,
+ * benchmarks/bench_step.py:101-109): while enabled, every labelled launch
+ * is bracketed by hipEvents on its stream.  Does not synchronise. */
+int slc_profile_enable(slc_plan* plan, int on);
+
+/* Synchronise on the recorded events and write one line per label,
+ * "label count total_ms\n", into buf (NUL-terminated); clears the records. */
+int slc_profile_report(slc_plan* plan, char* buf, int64_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SLC_H */

@@ -578,6 +578,16 @@ class ChannelOracle:
         self.u, self.v, self.w = (np.array(a, dtype=np.float64) for a in (u, v, w))
         self.p = None if p is None else np.array(p, dtype=np.float64)
 
+    def set_history(self, u_nm1, v_nm1, w_nm1, dt_prev):
+        """Adopt V^{n-1} (and the dt it was taken with) so the next step is a
+        BDF2 step; R^{n-1} = nu L_xy V^{n-1} exactly as the reference stores
+        it (solver.py:847-857 recomputes nothing: the stored R^{n-1} is this)."""
+        self.u_nm1 = tuple(np.array(a, dtype=np.float64) for a in (u_nm1, v_nm1, w_nm1))
+        a = (self.nx, self.ny, self.nz, self.dx, self.dy, self.nu)
+        self._Rxy_nm1 = (diffusion_xy_u(self.u_nm1[0], *a), diffusion_xy_v(self.u_nm1[1], *a),
+                         diffusion_xy_w(self.u_nm1[2], *a))
+        self._dt_prev_step = dt_prev
+
     def apply_bc_uvw(self):
         apply_bc_all(self.u, self.v, self.w, self.top_wall_bc_type)
 

@@ -1,0 +1,88 @@
+"""ctypes binding of libslc_hip.so (the C ABI declared in include/slc.h).
+
+The library is loaded from the package directory only; if it is missing the
+import of any GPU entry point raises -- there is no CPU fallback.
+"""
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libslc_hip.so")
+
+# Every symbol include/slc.h declares (checked by the CPU test suite).
+EXPORTS = (
+    "slc_version", "slc_last_error", "slc_plan_create", "slc_plan_destroy",
+    "slc_plan_workspace_bytes", "slc_apply_bc", "slc_departure", "slc_gather",
+    "slc_stencil", "slc_advect", "slc_bdf1_predict", "slc_bdf2_predict", "slc_zsolve", "slc_divergence",
+    "slc_poisson", "slc_project", "slc_project_step", "slc_bulk_velocity", "slc_bulk_force",
+    "slc_profile_enable", "slc_profile_report",
+)
+
+
+class GridDesc(ctypes.Structure):
+    """slc_grid_desc (include/slc.h)."""
+
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("Lx", ctypes.c_double), ("Ly", ctypes.c_double), ("Lz", ctypes.c_double),
+        ("gamma", ctypes.c_double), ("tanh_g", ctypes.c_double),
+        ("stretching", ctypes.c_int), ("order", ctypes.c_int), ("traj_order", ctypes.c_int),
+        ("n_traj_iters", ctypes.c_int), ("top_wall", ctypes.c_int),
+        ("nu", ctypes.c_double),
+        ("z_f", ctypes.POINTER(ctypes.c_double)), ("z_c", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+_lib = None
+
+
+def load():
+    """Load libslc_hip.so once; raise RuntimeError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"libslc_hip.so not found at {LIB_PATH}: build it first "
+            "(python -m slchannel_amd.build or __graft_entry__.build())"
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    i, d, i64p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+    sig = {
+        "slc_version": ([], i),
+        "slc_last_error": ([], ctypes.c_char_p),
+        "slc_plan_create": ([ctypes.POINTER(GridDesc), i, ctypes.POINTER(vp)], i),
+        "slc_plan_destroy": ([vp], i),
+        "slc_plan_workspace_bytes": ([vp, ctypes.POINTER(ctypes.c_int64)], i),
+        "slc_apply_bc": ([vp, vp, vp, vp, vp], i),
+        "slc_departure": ([vp, i, vp, vp, vp, d, vp, vp, vp, vp], i),
+        "slc_gather": ([vp, i, vp, i, vp, vp, vp, vp, i64p, vp], i),
+        "slc_stencil": ([vp, i, vp, vp, vp, vp, vp, vp, i64p, vp], i),
+        "slc_advect": ([vp, vp, vp, vp, vp, vp, vp, d, vp, vp, vp, i64p, vp], i),
+        "slc_bdf1_predict": ([vp, vp, vp, vp, d, vp, vp, vp, i64p, vp], i),
+        "slc_bdf2_predict": ([vp, vp, vp, vp, vp, vp, vp, d, vp, vp, vp, i64p, vp], i),
+        "slc_zsolve": ([vp, i, vp, d, d, vp], i),
+        "slc_divergence": ([vp, vp, vp, vp, vp, vp], i),
+        "slc_poisson": ([vp, vp, d, vp, vp], i),
+        "slc_project": ([vp, vp, vp, vp, vp, d, vp], i),
+        "slc_project_step": ([vp, vp, vp, vp, d, d, vp, vp], i),
+        "slc_bulk_velocity": ([vp, vp, vp, vp], i),
+        "slc_bulk_force": ([vp, vp, vp, vp, d, d, vp, vp], i),
+        "slc_profile_enable": ([vp, i], i),
+        "slc_profile_report": ([vp, ctypes.c_char_p, ctypes.c_int64], i),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def check(rc, what):
+    """Raise RuntimeError with the library's message on a non-zero status."""
+    if rc != 0:
+        msg = load().slc_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed (status {rc}): {msg}")

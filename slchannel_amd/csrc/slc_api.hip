@@ -1,0 +1,621 @@
+// slc_api.hip -- plan object and the extern "C" entry points of libslc_hip.so.
+//
+// The plan owns only small tables and the spectral work buffers; fields are
+// borrowed device pointers.  Host-side table construction follows the
+// reference formulas operation for operation (cited per table) so the HIP
+// path and the CPU oracle see the same numbers.
+#include <rocfft/rocfft.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/slc.h"
+#include "slc_internal.h"
+
+using namespace slc;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_OK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(SLC_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                               \
+    } while (0)
+
+#define FFT_OK(expr)                                                                   \
+    do {                                                                               \
+        rocfft_status s_ = (expr);                                                     \
+        if (s_ != rocfft_status_success)                                               \
+            return fail(SLC_EFFT, "%s failed: rocfft status %d", #expr, (int)s_);     \
+    } while (0)
+
+std::once_flag g_fft_once;
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
+        else prev = -1;
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace
+
+struct slc_plan {
+    int device = 0;
+    Grid g{};
+    PoissonTabs pt{};
+    int nx = 0, ny = 0, nz = 0, nky = 0;
+    double total_volume = 0.0;
+    std::vector<void*> allocs;
+    int64_t bytes = 0;
+    const double* d_dzc = nullptr;
+    const double* d_dzf = nullptr;
+    const double* d_cv = nullptr;
+    double* planes = nullptr;   // (nz, nx, ny) real
+    double* dhat = nullptr;     // (nz, nx, nky) complex
+    double* cpw = nullptr;      // (nz, nx, nky) Thomas c'
+    double* ztab = nullptr;     // 3 * (nz + 2) z-solve factor
+    double* partials = nullptr;
+    int max_partials = 0;
+    // opt-in launch timing: hipEvents recorded around each labelled launch
+    bool prof = false;
+    struct Rec {
+        const char* label;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    rocfft_plan fwd = nullptr, inv = nullptr;
+    rocfft_execution_info info = nullptr;
+    void* fft_work = nullptr;
+    size_t fft_work_bytes = 0;
+};
+
+namespace {
+
+int dev_alloc(slc_plan* p, size_t bytes, void** out) {
+    void* ptr = nullptr;
+    hipError_t e = hipMalloc(&ptr, bytes ? bytes : 8);
+    if (e != hipSuccess)
+        return fail(SLC_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    p->allocs.push_back(ptr);
+    p->bytes += (int64_t)bytes;
+    *out = ptr;
+    return SLC_OK;
+}
+
+int upload(slc_plan* p, const std::vector<double>& h, const double** out) {
+    void* d = nullptr;
+    int rc = dev_alloc(p, h.size() * sizeof(double), &d);
+    if (rc) return rc;
+    HIP_OK(hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    *out = static_cast<const double*>(d);
+    return SLC_OK;
+}
+
+// semilag.py:102-114: inv_denom[b, m] = 1 / prod_l (z[b+m] - z[b+l] + [l == m]),
+// product accumulated in increasing l.
+std::vector<double> denom_table(const std::vector<double>& zn, int order) {
+    int nb = (int)zn.size() - order + 1;
+    std::vector<double> t((size_t)(nb > 0 ? nb : 0) * order);
+    for (int b = 0; b < nb; ++b)
+        for (int m = 0; m < order; ++m) {
+            double prod = 0.0;
+            for (int l = 0; l < order; ++l) {
+                double d = (zn[b + m] - zn[b + l]) + (l == m ? 1.0 : 0.0);
+                prod = (l == 0) ? d : prod * d;
+            }
+            t[(size_t)b * order + m] = 1.0 / prod;
+        }
+    return t;
+}
+
+// torch.fft.fftfreq / rfftfreq with d = delta / (2 pi), then the modified
+// wavenumber (2/delta) sin(k delta / 2) squared (projection.py:47-60).
+std::vector<double> modified_k2(int n, double delta, bool half) {
+    const double two_pi = 2 * 3.141592653589793;
+    double d = delta / two_pi;
+    double val = 1.0 / ((double)n * d);
+    int m = half ? n / 2 + 1 : n;
+    std::vector<double> out(m);
+    for (int i = 0; i < m; ++i) {
+        int f = (half || i < (n + 1) / 2) ? i : i - n;
+        double k = (double)f * val;
+        double km = (2.0 / delta) * std::sin(k * delta / 2.0);
+        out[i] = km * km;
+    }
+    return out;
+}
+
+int check_plan(slc_plan* p) {
+    if (!p) return fail(SLC_EINVAL, "null plan");
+    return SLC_OK;
+}
+
+inline hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
+
+hipEvent_t pooled_event(slc_plan* p) {
+    if (!p->pool.empty()) {
+        hipEvent_t e = p->pool.back();
+        p->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Records events around one launch when profiling is on.
+struct Timed {
+    slc_plan* p;
+    const char* label;
+    hipStream_t s;
+    hipEvent_t a = nullptr;
+    Timed(slc_plan* p_, const char* l, void* st) : p(p_), label(l), s(S(st)) {
+        if (p->prof && (a = pooled_event(p))) (void)hipEventRecord(a, s);
+    }
+    ~Timed() {
+        if (!a) return;
+        hipEvent_t b = pooled_event(p);
+        if (!b) return;
+        (void)hipEventRecord(b, s);
+        p->recs.push_back({label, a, b});
+    }
+};
+
+const char* kLabel[3][6] = {
+    {"bdf2_far_u", "bdf2_near_u", "bdf1_u", "advect_u", "gather_u", "departure_u"},
+    {"bdf2_far_v", "bdf2_near_v", "bdf1_v", "advect_v", "gather_v", "departure_v"},
+    {"bdf2_far_w", "bdf2_near_w", "bdf1_w", "advect_w", "gather_w", "departure_w"},
+};
+const char* kZLabel[3] = {"zsolve_u", "zsolve_v", "zsolve_w"};
+
+int fft_exec(slc_plan* p, bool forward, void* stream) {
+    FFT_OK(rocfft_execution_info_set_stream(p->info, stream));
+    void* in[1] = {forward ? (void*)p->planes : (void*)p->dhat};
+    void* out[1] = {forward ? (void*)p->dhat : (void*)p->planes};
+    FFT_OK(rocfft_execute(forward ? p->fwd : p->inv, in, out, p->info));
+    return SLC_OK;
+}
+
+// planes (nz, nx, ny) -> Poisson solution in planes (unnormalised)
+int poisson_planes(slc_plan* p, void* stream) {
+    int rc;
+    {
+        Timed t(p, "fft_forward", stream);
+        rc = fft_exec(p, true, stream);
+    }
+    if (rc) return rc;
+    {
+        Timed t(p, "poisson_thomas", stream);
+        HIP_OK(run_poisson_thomas(p->pt, p->dhat, p->cpw, S(stream)));
+    }
+    Timed t(p, "fft_inverse", stream);
+    return fft_exec(p, false, stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int slc_version(void) { return 100; }
+
+const char* slc_last_error(void) { return g_err.c_str(); }
+
+int slc_plan_destroy(slc_plan* p) {
+    if (!p) return SLC_OK;
+    DeviceGuard dg(p->device);
+    if (p->fwd) rocfft_plan_destroy(p->fwd);
+    if (p->inv) rocfft_plan_destroy(p->inv);
+    if (p->info) rocfft_execution_info_destroy(p->info);
+    for (auto& r : p->recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (hipEvent_t e : p->pool) (void)hipEventDestroy(e);
+    for (void* a : p->allocs) (void)hipFree(a);
+    delete p;
+    return SLC_OK;
+}
+
+int slc_plan_create(const slc_grid_desc* d, int device, slc_plan** out) {
+    if (!d || !out) return fail(SLC_EINVAL, "null argument");
+    *out = nullptr;
+    if (d->nx < 1 || d->ny < 2 || d->nz < 2)
+        return fail(SLC_EINVAL, "grid too small (nx=%d ny=%d nz=%d)", d->nx, d->ny, d->nz);
+    if (d->order != 4 && d->order != 6)
+        return fail(SLC_EINVAL, "sl.interp_order must be 4 or 6, got %d", d->order);
+    if (d->traj_order != 2)
+        return fail(SLC_EINVAL, "traj_interp_order %d not supported (trilinear = 2 only)",
+                    d->traj_order);
+    if (d->nz + 1 < d->order)
+        return fail(SLC_EINVAL, "nz=%d too small for interpolation order %d", d->nz, d->order);
+    if (d->stretching != 0 && d->stretching != 1)
+        return fail(SLC_EINVAL, "stretching must be 0 (symmetric) or 1 (bottom)");
+    if (d->n_traj_iters < 0) return fail(SLC_EINVAL, "n_traj_iters must be >= 0");
+    if (!d->z_f || !d->z_c) return fail(SLC_EINVAL, "z_f / z_c host tables required");
+
+    DeviceGuard dg(device);
+    std::call_once(g_fft_once, [] { rocfft_setup(); });
+    slc_plan* p = new slc_plan();
+    p->device = device;
+    const int nx = d->nx, ny = d->ny, nz = d->nz, P = d->order;
+    p->nx = nx;
+    p->ny = ny;
+    p->nz = nz;
+    p->nky = ny / 2 + 1;
+    const double dx = d->Lx / nx, dy = d->Ly / ny;
+
+    std::vector<double> zf(d->z_f, d->z_f + nz + 1), zc(d->z_c, d->z_c + nz + 2);
+    std::vector<double> dzf(nz), dzc(nz + 1), cv(nz);
+    for (int k = 0; k < nz; ++k) dzf[k] = zf[k + 1] - zf[k];      // utils.py:58
+    for (int k = 0; k <= nz; ++k) dzc[k] = zc[k + 1] - zc[k];     // utils.py:59
+    for (int k = 0; k < nz; ++k) cv[k] = dx * dy * dzf[k];        // solver.py:317
+    p->total_volume = d->Lx * d->Ly * d->Lz;                      // solver.py:321
+
+    Grid& g = p->g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.order = P;
+    g.n_iters = d->n_traj_iters;
+    g.symmetric = d->stretching == 0 ? 1 : 0;
+    g.top_neumann = d->top_wall == 1 ? 1 : 0;
+    g.dx = dx;
+    g.dy = dy;
+    g.Lz = d->Lz;
+    g.gamma = d->gamma;
+    g.tanh_g = d->tanh_g;
+    const double eps_z = 1e-12 * d->Lz;  // semilag.py:245-247
+    g.z_lo = zf[0] + eps_z;
+    g.z_hi = zf[nz] - eps_z;
+    g.nu = d->nu;
+
+    int rc = 0;
+    const double* tmp = nullptr;
+#define UP(vec, dst)                         \
+    do {                                     \
+        rc = upload(p, (vec), &tmp);         \
+        if (rc) {                            \
+            slc_plan_destroy(p);             \
+            return rc;                       \
+        }                                    \
+        dst = tmp;                           \
+    } while (0)
+    UP(zc, g.z_c);
+    UP(zf, g.z_f);
+    UP(denom_table(zc, 2), g.dinv_c2);
+    UP(denom_table(zf, 2), g.dinv_f2);
+    UP(denom_table(zc, P), g.dinv_cP);
+    UP(denom_table(zf, P), g.dinv_fP);
+    UP(dzc, p->d_dzc);
+    UP(dzf, p->d_dzf);
+    UP(cv, p->d_cv);
+
+    // Poisson tables (projection.py:65-103) as 1-D factors
+    std::vector<double> cl(nz), cr(nz), cs(nz);
+    for (int k = 0; k < nz; ++k) {
+        cl[k] = 1.0 / (dzc[k] * dzf[k]);
+        cr[k] = 1.0 / (dzc[k + 1] * dzf[k]);
+        cs[k] = cl[k] + cr[k];
+    }
+    PoissonTabs& pt = p->pt;
+    pt.nx = nx;
+    pt.ny = ny;
+    pt.nz = nz;
+    pt.nky = p->nky;
+    UP(modified_k2(nx, dx, false), pt.kxsq);
+    UP(modified_k2(ny, dy, true), pt.kysq);
+    UP(cl, pt.cl);
+    UP(cr, pt.cr);
+    UP(cs, pt.cs);
+#undef UP
+
+    const long nplane = (long)nx * ny, nmodes = (long)nx * p->nky;
+    void* ptr = nullptr;
+    if ((rc = dev_alloc(p, sizeof(double) * nplane * nz, &ptr))) { slc_plan_destroy(p); return rc; }
+    p->planes = (double*)ptr;
+    if ((rc = dev_alloc(p, 2 * sizeof(double) * nmodes * nz, &ptr))) { slc_plan_destroy(p); return rc; }
+    p->dhat = (double*)ptr;
+    if ((rc = dev_alloc(p, sizeof(double) * nmodes * nz, &ptr))) { slc_plan_destroy(p); return rc; }
+    p->cpw = (double*)ptr;
+    if ((rc = dev_alloc(p, sizeof(double) * 3 * (nz + 2), &ptr))) { slc_plan_destroy(p); return rc; }
+    p->ztab = (double*)ptr;
+    p->max_partials = bulk_partials_needed(g);
+    if ((rc = dev_alloc(p, sizeof(double) * p->max_partials, &ptr))) { slc_plan_destroy(p); return rc; }
+    p->partials = (double*)ptr;
+
+    // rocFFT: nz batched 2-D transforms of contiguous (nx, ny) planes; the
+    // halved (Hermitian) dimension is y, as torch.fft.rfft2 over dims (0, 1).
+    size_t lengths[2] = {(size_t)ny, (size_t)nx};  // fastest first
+    rocfft_status fs;
+    fs = rocfft_plan_create(&p->fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                            rocfft_precision_double, 2, lengths, (size_t)nz, nullptr);
+    if (fs != rocfft_status_success) {
+        slc_plan_destroy(p);
+        return fail(SLC_EFFT, "rocfft_plan_create(forward) status %d", (int)fs);
+    }
+    fs = rocfft_plan_create(&p->inv, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
+                            rocfft_precision_double, 2, lengths, (size_t)nz, nullptr);
+    if (fs != rocfft_status_success) {
+        slc_plan_destroy(p);
+        return fail(SLC_EFFT, "rocfft_plan_create(inverse) status %d", (int)fs);
+    }
+    size_t w1 = 0, w2 = 0;
+    rocfft_plan_get_work_buffer_size(p->fwd, &w1);
+    rocfft_plan_get_work_buffer_size(p->inv, &w2);
+    p->fft_work_bytes = w1 > w2 ? w1 : w2;
+    rocfft_execution_info_create(&p->info);
+    if (p->fft_work_bytes) {
+        if ((rc = dev_alloc(p, p->fft_work_bytes, &ptr))) { slc_plan_destroy(p); return rc; }
+        p->fft_work = ptr;
+        rocfft_execution_info_set_work_buffer(p->info, p->fft_work, p->fft_work_bytes);
+    }
+    *out = p;
+    return SLC_OK;
+}
+
+int slc_plan_workspace_bytes(const slc_plan* p, int64_t* out) {
+    if (!p || !out) return fail(SLC_EINVAL, "null argument");
+    *out = p->bytes;
+    return SLC_OK;
+}
+
+int slc_apply_bc(slc_plan* p, double* u, double* v, double* w, void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    DeviceGuard dg(p->device);
+    Timed t(p, "ghost_fill", stream);
+    HIP_OK(run_bc(p->g, u, v, w, S(stream)));
+    return SLC_OK;
+}
+
+int slc_departure(slc_plan* p, int comp, const double* um, const double* vm, const double* wm,
+                  double dt, double* xd, double* yd, double* zd, void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (comp < 0 || comp > 2) return fail(SLC_EINVAL, "comp must be 0, 1 or 2");
+    if (!um || !vm || !wm || !xd || !yd || !zd) return fail(SLC_EINVAL, "null array");
+    DeviceGuard dg(p->device);
+    TrajVel V{{um, vm, wm}, {nullptr, nullptr, nullptr}};
+    Timed t(p, kLabel[comp][5], stream);
+    HIP_OK(run_departure(p->g, comp, V, dt, xd, yd, zd, S(stream)));
+    return SLC_OK;
+}
+
+int slc_gather(slc_plan* p, int comp, const double* field, int layout, const double* xd,
+               const double* yd, const double* zd, double* out, int64_t* nclamp, void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (comp < 0 || comp > 2) return fail(SLC_EINVAL, "comp must be 0, 1 or 2");
+    if (layout != 0 && layout != 1) return fail(SLC_EINVAL, "layout must be 0 or 1");
+    if (!field || !xd || !yd || !zd || !out) return fail(SLC_EINVAL, "null array");
+    DeviceGuard dg(p->device);
+    long fsx = 0, fsy = 0;
+    if (layout == 1) {
+        long NZ = comp == 2 ? p->nz + 1 : p->nz + 2;
+        fsy = NZ;
+        fsx = (long)p->ny * NZ;
+    }
+    Timed t(p, kLabel[comp][4], stream);
+    HIP_OK(run_gather(p->g, comp, field, fsx, fsy, 0, 0, xd, yd, zd, out,
+                      reinterpret_cast<unsigned long long*>(nclamp), S(stream)));
+    return SLC_OK;
+}
+
+int slc_stencil(slc_plan* p, int comp, const double* xd, const double* yd, const double* zd,
+                int32_t* ix, int32_t* iy, int32_t* k0, int64_t* nclamp, void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (comp < 0 || comp > 2) return fail(SLC_EINVAL, "comp must be 0, 1 or 2");
+    if (!xd || !yd || !zd || !ix || !iy || !k0) return fail(SLC_EINVAL, "null array");
+    DeviceGuard dg(p->device);
+    HIP_OK(run_stencil(p->g, comp, xd, yd, zd, ix, iy, k0,
+                       reinterpret_cast<unsigned long long*>(nclamp), S(stream)));
+    return SLC_OK;
+}
+
+int slc_advect(slc_plan* p, const double* u, const double* v, const double* w, const double* um,
+               const double* vm, const double* wm, double dt, double* ustar, double* vstar,
+               double* wstar, int64_t* nclamp, void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (!u || !v || !w || !um || !vm || !wm || !ustar || !vstar || !wstar)
+        return fail(SLC_EINVAL, "null array");
+    DeviceGuard dg(p->device);
+    TrajVel V{{um, vm, wm}, {nullptr, nullptr, nullptr}};
+    const double* F[3] = {u, v, w};
+    double* O[3] = {ustar, vstar, wstar};
+    auto* ncl = reinterpret_cast<unsigned long long*>(nclamp);
+    for (int c = 0; c < 3; ++c) {
+        Timed t(p, kLabel[c][3], stream);
+        HIP_OK(run_advect(p->g, c, V, F[c], dt, O[c], ncl, S(stream)));
+    }
+    return SLC_OK;
+}
+
+int slc_bdf1_predict(slc_plan* p, const double* u, const double* v, const double* w, double dt,
+                     double* ustar, double* vstar, double* wstar, int64_t* nclamp,
+                     void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (!u || !v || !w || !ustar || !vstar || !wstar) return fail(SLC_EINVAL, "null array");
+    DeviceGuard dg(p->device);
+    TrajVel V{{u, v, w}, {nullptr, nullptr, nullptr}};
+    const double* F[3] = {u, v, w};
+    double* O[3] = {ustar, vstar, wstar};
+    auto* ncl = reinterpret_cast<unsigned long long*>(nclamp);
+    for (int c = 0; c < 3; ++c) {
+        Timed t(p, kLabel[c][2], stream);
+        HIP_OK(run_bdf1(p->g, c, V, F[c], dt, O[c], ncl, S(stream)));
+    }
+    return SLC_OK;
+}
+
+int slc_bdf2_predict(slc_plan* p, const double* u, const double* v, const double* w,
+                     const double* unm1, const double* vnm1, const double* wnm1, double dt,
+                     double* ustar, double* vstar, double* wstar, int64_t* nclamp,
+                     void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (!u || !v || !w || !unm1 || !vnm1 || !wnm1 || !ustar || !vstar || !wstar)
+        return fail(SLC_EINVAL, "null array");
+    DeviceGuard dg(p->device);
+    TrajVel V{{u, v, w}, {unm1, vnm1, wnm1}};
+    const double* Fn[3] = {u, v, w};
+    const double* Fm[3] = {unm1, vnm1, wnm1};
+    double* O[3] = {ustar, vstar, wstar};
+    auto* ncl = reinterpret_cast<unsigned long long*>(nclamp);
+    for (int pass = 0; pass < 2; ++pass)
+        for (int c = 0; c < 3; ++c) {
+            Timed t(p, kLabel[c][pass], stream);
+            HIP_OK(run_bdf2(p->g, c, pass, V, Fn[c], Fm[c], dt, O[c], ncl, S(stream)));
+        }
+    return SLC_OK;
+}
+
+int slc_zsolve(slc_plan* p, int comp, double* field, double dt, double theta, void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (comp < 0 || comp > 2) return fail(SLC_EINVAL, "comp must be 0, 1 or 2");
+    if (!field) return fail(SLC_EINVAL, "null array");
+    if (!(theta > 0.0 && theta <= 1.0)) return fail(SLC_EINVAL, "theta must be in (0, 1]");
+    if (comp == 2 && p->nz < 3) return fail(SLC_EINVAL, "w z-solve needs nz >= 3");
+    DeviceGuard dg(p->device);
+    Timed t(p, kZLabel[comp], stream);
+    HIP_OK(run_zsolve(p->g, comp, field, dt, theta, p->d_dzc, p->d_dzf, p->ztab, S(stream)));
+    return SLC_OK;
+}
+
+int slc_divergence(slc_plan* p, const double* u, const double* v, const double* w, double* div,
+                   void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (!u || !v || !w || !div) return fail(SLC_EINVAL, "null array");
+    DeviceGuard dg(p->device);
+    HIP_OK(run_div_field(p->g, u, v, w, p->d_dzf, div, S(stream)));
+    return SLC_OK;
+}
+
+int slc_poisson(slc_plan* p, const double* div, double rhs_scale, double* pr, void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (!div || !pr) return fail(SLC_EINVAL, "null array");
+    DeviceGuard dg(p->device);
+    HIP_OK(run_field_to_planes(p->g, div, rhs_scale, p->planes, S(stream)));
+    if (int rc = poisson_planes(p, stream)) return rc;
+    const double inv_n = 1.0 / ((double)p->nx * (double)p->ny);
+    HIP_OK(run_project_planes(p->g, nullptr, nullptr, nullptr, p->planes, inv_n, 0.0, p->d_dzc,
+                              pr, S(stream)));
+    HIP_OK(run_bc_p(p->g, pr, S(stream)));
+    return SLC_OK;
+}
+
+int slc_project(slc_plan* p, double* u, double* v, double* w, const double* pr, double dt,
+                void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (!u || !v || !w || !pr) return fail(SLC_EINVAL, "null array");
+    DeviceGuard dg(p->device);
+    HIP_OK(run_project_p(p->g, u, v, w, pr, dt, p->d_dzc, S(stream)));
+    return SLC_OK;
+}
+
+int slc_project_step(slc_plan* p, double* u, double* v, double* w, double rhs_scale,
+                     double dt_proj, double* pr, void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (!u || !v || !w) return fail(SLC_EINVAL, "null array");
+    DeviceGuard dg(p->device);
+    {
+        Timed t(p, "divergence_planes", stream);
+        HIP_OK(run_div_planes(p->g, u, v, w, p->d_dzf, rhs_scale, p->planes, S(stream)));
+    }
+    if (int rc = poisson_planes(p, stream)) return rc;
+    const double inv_n = 1.0 / ((double)p->nx * (double)p->ny);
+    {
+        Timed t(p, "project_planes", stream);
+        HIP_OK(run_project_planes(p->g, u, v, w, p->planes, inv_n, dt_proj, p->d_dzc, pr,
+                                  S(stream)));
+    }
+    Timed t(p, "ghost_fill_p_uvw", stream);
+    if (pr) HIP_OK(run_bc_p(p->g, pr, S(stream)));
+    HIP_OK(run_bc(p->g, u, v, w, S(stream)));
+    return SLC_OK;
+}
+
+int slc_bulk_velocity(slc_plan* p, const double* u, double* out, void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (!u || !out) return fail(SLC_EINVAL, "null array");
+    DeviceGuard dg(p->device);
+    HIP_OK(run_bulk_velocity(p->g, u, p->d_cv, p->total_volume, p->partials, p->max_partials, out,
+                             S(stream)));
+    return SLC_OK;
+}
+
+int slc_bulk_force(slc_plan* p, double* u, double* v, double* w, double U_bulk, double dt,
+                   double* out, void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (!u || !out) return fail(SLC_EINVAL, "null array");
+    if (!(dt != 0.0)) return fail(SLC_EINVAL, "dt must be non-zero");
+    DeviceGuard dg(p->device);
+    Timed t(p, "bulk_force", stream);
+    HIP_OK(run_bulk(p->g, u, v, w, p->d_cv, p->total_volume, U_bulk, dt, 1, p->partials,
+                    p->max_partials, out, S(stream)));
+    return SLC_OK;
+}
+
+int slc_profile_enable(slc_plan* p, int on) {
+    if (int rc = check_plan(p)) return rc;
+    p->prof = on != 0;
+    return SLC_OK;
+}
+
+int slc_profile_report(slc_plan* p, char* buf, int64_t len) {
+    if (int rc = check_plan(p)) return rc;
+    if (!buf || len <= 0) return fail(SLC_EINVAL, "null/empty buffer");
+    DeviceGuard dg(p->device);
+    struct Acc {
+        std::string label;
+        double total = 0.0;
+        long n = 0;
+    };
+    std::vector<Acc> acc;
+    for (auto& r : p->recs) {
+        HIP_OK(hipEventSynchronize(r.b));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, r.a, r.b));
+        size_t i = 0;
+        for (; i < acc.size(); ++i)
+            if (acc[i].label == r.label) break;
+        if (i == acc.size()) acc.push_back({r.label, 0.0, 0});
+        acc[i].total += ms;
+        acc[i].n += 1;
+        p->pool.push_back(r.a);
+        p->pool.push_back(r.b);
+    }
+    p->recs.clear();
+    std::string out;
+    char line[160];
+    for (auto& a : acc) {
+        snprintf(line, sizeof line, "%s %ld %.6f\n", a.label.c_str(), a.n, a.total);
+        out += line;
+    }
+    if ((int64_t)out.size() + 1 > len) return fail(SLC_EINVAL, "report buffer too small");
+    memcpy(buf, out.c_str(), out.size() + 1);
+    return SLC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,382 @@
+// slc_device.h -- device-side grid description and the semi-Lagrangian
+// building blocks shared by the gfx950 kernels.
+//
+// Arrays are the ghost-shaped staggered fields of the reference solver
+// (C order, z fastest):
+//   u (nx+1, ny+2, nz+2)   x-faces, y centres, z centres (+ghosts)
+//   v (nx+2, ny+1, nz+2)   x centres, y-faces, z centres (+ghosts)
+//   w (nx+2, ny+2, nz+1)   x centres, y centres, z faces (walls included)
+//   p (nx+2, ny+2, nz+2)
+// The interpolation "node buffer" of a component (semilag.py:387-394: one x-y
+// period plus the whole z column) is read in place through NodeMap, so the
+// reference's six node-buffer copies per advect call do not exist here.
+//
+// Floating point: every routine on the path arrival -> departure foot ->
+// stencil index is written in the reference's operation order and this file
+// is compiled with -ffp-contract=off, so the feet and the stencil indices are
+// the same IEEE-754 results as the CPU oracle's.  Fused multiply-adds appear
+// only where written explicitly (fma), i.e. in the high-order accumulation
+// whose tolerance is a relative 1e-13, not bit-exactness.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace slc {
+
+struct Grid {
+    int nx, ny, nz;
+    int order;        // field interpolation order P (4 | 6)
+    int n_iters;      // iterated-midpoint fixed-point iterations
+    int symmetric;    // 1: symmetric tanh map, 0: 'bottom'
+    int top_neumann;  // top wall: 0 no-slip (odd ghost), 1 free-slip (even ghost)
+    double dx, dy, Lz, gamma, tanh_g, z_lo, z_hi, nu;
+    const double* z_c;      // nz+2 centres incl. ghost centres
+    const double* z_f;      // nz+1 faces
+    const double* dinv_c2;  // order-2 inverse Lagrange denominators, centre nodes
+    const double* dinv_f2;  // order-2, face nodes
+    const double* dinv_cP;  // order-P, centre nodes  ((nz+2)-P+1, P)
+    const double* dinv_fP;  // order-P, face nodes    ((nz+1)-P+1, P)
+};
+
+// Component specs (semilag.py:282-286 node grids; solver.py:70-107 shapes).
+template <int C>
+struct Comp;
+
+template <>
+struct Comp<0> {  // u
+    static constexpr double shift_x = 0.0, shift_y = -0.5;
+    static constexpr bool centers = true;
+    static constexpr int ox = 0, oy = 1;
+    __device__ static long SY(const Grid& g) { return g.ny + 2; }
+    __device__ static long SZ(const Grid& g) { return g.nz + 2; }
+    __device__ static int NZ(const Grid& g) { return g.nz + 2; }
+    __device__ static int NZI(const Grid& g) { return g.nz; }
+};
+template <>
+struct Comp<1> {  // v
+    static constexpr double shift_x = -0.5, shift_y = 0.0;
+    static constexpr bool centers = true;
+    static constexpr int ox = 1, oy = 0;
+    __device__ static long SY(const Grid& g) { return g.ny + 1; }
+    __device__ static long SZ(const Grid& g) { return g.nz + 2; }
+    __device__ static int NZ(const Grid& g) { return g.nz + 2; }
+    __device__ static int NZI(const Grid& g) { return g.nz; }
+};
+template <>
+struct Comp<2> {  // w
+    static constexpr double shift_x = -0.5, shift_y = -0.5;
+    static constexpr bool centers = false;
+    static constexpr int ox = 1, oy = 1;
+    __device__ static long SY(const Grid& g) { return g.ny + 2; }
+    __device__ static long SZ(const Grid& g) { return g.nz + 1; }
+    __device__ static int NZ(const Grid& g) { return g.nz + 1; }
+    __device__ static int NZI(const Grid& g) { return g.nz - 1; }
+};
+
+template <int C>
+__device__ inline const double* znodes(const Grid& g) {
+    return Comp<C>::centers ? g.z_c : g.z_f;
+}
+
+__device__ inline double clampd(double x, double lo, double hi) {
+    double y = x < lo ? lo : x;  // torch.clamp: min(max(x, lo), hi)
+    return y > hi ? hi : y;
+}
+
+__device__ inline int pymod(long a, int n) {
+    long r = a % n;
+    return (int)(r < 0 ? r + n : r);
+}
+
+// Continuous face index kf (semilag.py:400-411).
+__device__ inline double face_coord(const Grid& g, double z) {
+    if (g.symmetric) {
+        double arg = (2.0 * z / g.Lz - 1.0) * g.tanh_g;
+        arg = clampd(arg, -1.0 + 1e-15, 1.0 - 1e-15);
+        double xi = atanh(arg) / g.gamma;
+        return (xi + 1.0) * (0.5 * (double)g.nz);
+    } else {
+        double arg = (1.0 - z / g.Lz) * g.tanh_g;
+        arg = clampd(arg, -1.0 + 1e-15, 1.0 - 1e-15);
+        double xi = 1.0 - atanh(arg) / g.gamma;
+        return xi * (double)g.nz;
+    }
+}
+
+// Face-interval index c = clamp(floor(kf), 0, nz-1) (semilag.py:415-416).
+__device__ inline int face_cell(const Grid& g, double z) {
+    double kf = face_coord(g, z);
+    double f = floor(kf);
+    long c = (long)f;
+    if (c < 0) c = 0;
+    if (c > g.nz - 1) c = g.nz - 1;
+    return (int)c;
+}
+
+// Interval index m0 of the component's z nodes containing z (semilag.py:413-420).
+template <int C>
+__device__ inline int locate_from_cell(const Grid& g, int c, double z) {
+    if (!Comp<C>::centers) return c;
+    return c + (z >= g.z_c[c + 1] ? 1 : 0);
+}
+
+// Uniform Lagrange weights on offsets l-(P/2-1) (semilag.py:426-439 with the
+// inverse denominators of semilag.py:117-127), product taken in increasing l.
+template <int P>
+__device__ inline void uniform_weights(double t, double* w) {
+    constexpr int h = P / 2 - 1;
+    double d[P];
+#pragma unroll
+    for (int l = 0; l < P; ++l) d[l] = t - (double)(l - h);
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        double den = 1.0;
+#pragma unroll
+        for (int l = 0; l < P; ++l)
+            if (l != m) den *= (double)((m - h) - (l - h));
+        double num = 0.0;
+        bool first = true;
+#pragma unroll
+        for (int l = 0; l < P; ++l) {
+            if (l == m) continue;
+            num = first ? d[l] : num * d[l];
+            first = false;
+        }
+        w[m] = num * (1.0 / den);
+    }
+}
+
+// Nonuniform z weights against the actual nodes (semilag.py:441-454).
+template <int P>
+__device__ inline void z_weights(const double* zn, const double* dinv, double z, int k0,
+                                 double* w) {
+    double d[P];
+#pragma unroll
+    for (int l = 0; l < P; ++l) d[l] = z - zn[k0 + l];
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        double num = 0.0;
+        bool first = true;
+#pragma unroll
+        for (int l = 0; l < P; ++l) {
+            if (l == m) continue;
+            num = first ? d[l] : num * d[l];
+            first = false;
+        }
+        w[m] = num * dinv[k0 * P + m];
+    }
+}
+
+// Linear address of node (ib, jb, k) of component C inside its ghost-shaped array.
+template <int C>
+__device__ inline long node_addr(const Grid& g, int ib, int jb, int k) {
+    return ((long)(ib + Comp<C>::ox) * Comp<C>::SY(g) + (jb + Comp<C>::oy)) * Comp<C>::SZ(g) + k;
+}
+
+// Arrival point of flat interior index n (z fastest) of component C
+// (semilag.py:292-307).  Also returns the ghost-array indices (I, J, K).
+template <int C>
+__device__ inline void arrival(const Grid& g, long n, double& xa, double& ya, double& za, int& I,
+                               int& J, int& K) {
+    const int nzi = Comp<C>::NZI(g);
+    int k = (int)(n % nzi);
+    long r = n / nzi;
+    int j = (int)(r % g.ny);
+    int i = (int)(r / g.ny);
+    double ar = (double)(i + 1), aj = (double)(j + 1);
+    if (C == 0) {
+        xa = ar * g.dx;
+        ya = (aj - 0.5) * g.dy;
+        za = g.z_c[k + 1];
+    } else if (C == 1) {
+        xa = (ar - 0.5) * g.dx;
+        ya = aj * g.dy;
+        za = g.z_c[k + 1];
+    } else {
+        xa = (ar - 0.5) * g.dx;
+        ya = (aj - 0.5) * g.dy;
+        za = g.z_f[k + 1];
+    }
+    I = i + 1;
+    J = j + 1;
+    K = k + 1;
+}
+
+// Trajectory velocity accessor: either a stored field or U* = 2a - b formed
+// on the fly (solver.py:866-868 forms the same single rounding).
+struct TrajVel {
+    const double* a[3];
+    const double* b[3];  // null => U* = a
+    __device__ inline double at(int c, long idx) const {
+        return b[c] ? 2.0 * a[c][idx] - b[c][idx] : a[c][idx];
+    }
+};
+
+// Trilinear sample of trajectory component C at (x, y, zc) where the caller
+// has already clamped z (semilag.py:512-528 with _build_iw_impl at order 2 and
+// _gather_interp's sequential (i, j, k) sum).  `c` is the face cell of zc.
+template <int C>
+__device__ inline double trilinear(const Grid& g, const TrajVel& V, double xdx, double ydy,
+                                   double zc, int cell) {
+    double sx = xdx + Comp<C>::shift_x;
+    double ix0 = floor(sx);
+    double tx = sx - ix0;
+    double sy = ydy + Comp<C>::shift_y;
+    double iy0 = floor(sy);
+    double ty = sy - iy0;
+    int ixa = pymod((long)ix0, g.nx), ixb = pymod((long)ix0 + 1, g.nx);
+    int iya = pymod((long)iy0, g.ny), iyb = pymod((long)iy0 + 1, g.ny);
+    double wx[2], wy[2], wz[2];
+    // order-2 uniform weights: (t - 1) * (-1), t * 1
+    wx[0] = (tx - 1.0) * (-1.0);
+    wx[1] = tx * 1.0;
+    wy[0] = (ty - 1.0) * (-1.0);
+    wy[1] = ty * 1.0;
+    int m0 = locate_from_cell<C>(g, cell, zc);
+    int k0 = m0;  // offset P/2-1 = 0 at order 2
+    const int NZ = Comp<C>::NZ(g);
+    if (k0 > NZ - 2) k0 = NZ - 2;
+    if (k0 < 0) k0 = 0;
+    const double* zn = znodes<C>(g);
+    const double* dinv = Comp<C>::centers ? g.dinv_c2 : g.dinv_f2;
+    double d0 = zc - zn[k0], d1 = zc - zn[k0 + 1];
+    wz[0] = d1 * dinv[k0 * 2 + 0];
+    wz[1] = d0 * dinv[k0 * 2 + 1];
+    int ix[2] = {ixa, ixb}, iy[2] = {iya, iyb};
+    double out = 0.0;
+    bool first = true;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            long base = node_addr<C>(g, ix[i], iy[j], k0);
+            double wij = wx[i] * wy[j];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                double term = (wij * wz[k]) * V.at(C, base + k);
+                out = first ? term : out + term;
+                first = false;
+            }
+        }
+    }
+    return out;
+}
+
+// Departure foot of one arrival point (semilag.py:534-552): K iterations of
+// x_m <- x_a - (dt/2) U*(x_m), z_m clamped; returns x_d = 2 x_m - x_a with z
+// unclamped.  The three trajectory samples at one point share x/dx, y/dy and
+// the face cell of z (the reference recomputes them with identical results).
+template <int DUMMY = 0>
+__device__ inline void foot(const Grid& g, const TrajVel& V, double xa, double ya, double za,
+                            double half_dt, double& xd, double& yd, double& zd) {
+    double xm = xa, ym = ya, zm = za;
+    for (int it = 0; it < g.n_iters; ++it) {
+        double zc = clampd(zm, g.z_lo, g.z_hi);
+        double xdx = xm / g.dx;
+        double ydy = ym / g.dy;
+        int cell = face_cell(g, zc);
+        double us = trilinear<0>(g, V, xdx, ydy, zc, cell);
+        double vs = trilinear<1>(g, V, xdx, ydy, zc, cell);
+        double ws = trilinear<2>(g, V, xdx, ydy, zc, cell);
+        xm = xa - half_dt * us;
+        ym = ya - half_dt * vs;
+        zm = clampd(za - half_dt * ws, g.z_lo, g.z_hi);
+    }
+    xd = 2.0 * xm - xa;
+    yd = 2.0 * ym - ya;
+    zd = 2.0 * zm - za;
+}
+
+// Order-P stencil of a (clamped) foot (semilag.py:471-500).
+template <int C, int P>
+struct Stencil {
+    int ix[P], iy[P];
+    int k0;
+    double wx[P], wy[P], wz[P];
+};
+
+template <int C, int P>
+__device__ inline void build_stencil(const Grid& g, double x, double y, double z,
+                                     Stencil<C, P>& s) {
+    constexpr int h = P / 2 - 1;
+    double sx = x / g.dx + Comp<C>::shift_x;
+    double ix0 = floor(sx);
+    double tx = sx - ix0;
+    double sy = y / g.dy + Comp<C>::shift_y;
+    double iy0 = floor(sy);
+    double ty = sy - iy0;
+    long bx = (long)ix0 - h, by = (long)iy0 - h;
+    int x0 = pymod(bx, g.nx), y0 = pymod(by, g.ny);
+#pragma unroll
+    for (int l = 0; l < P; ++l) {
+        int xi = x0 + l;
+        while (xi >= g.nx) xi -= g.nx;  // at most one pass unless nx < P
+        s.ix[l] = xi;
+        int yi = y0 + l;
+        while (yi >= g.ny) yi -= g.ny;
+        s.iy[l] = yi;
+    }
+    uniform_weights<P>(tx, s.wx);
+    uniform_weights<P>(ty, s.wy);
+    int m0 = locate_from_cell<C>(g, face_cell(g, z), z);
+    int k0 = m0 - h;
+    const int NZ = Comp<C>::NZ(g);
+    if (k0 < 0) k0 = 0;
+    if (k0 > NZ - P) k0 = NZ - P;
+    s.k0 = k0;
+    z_weights<P>(znodes<C>(g), Comp<C>::centers ? g.dinv_cP : g.dinv_fP, z, k0, s.wz);
+}
+
+// Staged tensor-product contraction (z, then y, then x) with FMAs.
+template <int C, int P>
+__device__ inline double gather(const Grid& g, const double* __restrict__ F,
+                                const Stencil<C, P>& s) {
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        double ai = 0.0;
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const double* col = F + node_addr<C>(g, s.ix[i], s.iy[j], s.k0);
+            double sj = 0.0;
+#pragma unroll
+            for (int k = 0; k < P; ++k) sj = fma(s.wz[k], col[k], sj);
+            ai = fma(s.wy[j], sj, ai);
+        }
+        acc = fma(s.wx[i], ai, acc);
+    }
+    return acc;
+}
+
+// Explicit xy-diffusion nu*(d2x + d2y) at ghost index (I, J, K)
+// (operators.py:293-367; periodic extension for u in x and v in y).
+template <int C>
+__device__ inline double rhs_xy(const Grid& g, const double* __restrict__ F, int I, int J, int K,
+                                double dx2, double dy2) {
+    const long SY = Comp<C>::SY(g), SZ = Comp<C>::SZ(g);
+    int Ip = I + 1, Jp = J + 1;
+    if (C == 0 && Ip == g.nx + 1) Ip = 1;
+    if (C == 1 && Jp == g.ny + 1) Jp = 1;
+    auto at = [&](int i, int j) { return F[((long)i * SY + j) * SZ + K]; };
+    double c = at(I, J);
+    double d2x = ((at(Ip, J) - 2.0 * c) + at(I - 1, J)) / dx2;
+    double d2y = ((at(I, Jp) - 2.0 * c) + at(I, J - 1)) / dy2;
+    return g.nu * (d2x + d2y);
+}
+
+// blockIdx -> tile remap that keeps consecutive tiles on one XCD (blocks are
+// dispatched round-robin over the 8 XCDs; each XCD has its own L2).
+__device__ inline long xcd_block(long nblocks) {
+    long b = blockIdx.x;
+    const long nx = 8;
+    long per = nblocks / nx;
+    long rem = nblocks % nx;
+    long x = b % nx, q = b / nx;
+    // blocks of XCD x occupy logical range [start_x, start_x + count_x)
+    long start = x * per + (x < rem ? x : rem);
+    long count = per + (x < rem ? 1 : 0);
+    return q < count ? start + q : b;
+}
+
+}  // namespace slc

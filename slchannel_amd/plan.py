@@ -1,0 +1,256 @@
+"""HipPlan: the Python face of one slc_plan (libslc_hip.so).
+
+Owns the native plan for one grid and exposes the step's operators on torch
+float64 CUDA(HIP) tensors.  Every argument is validated on the host (device,
+dtype, contiguity, exact shape) before any kernel is launched, so a kernel
+never sees a buffer of the wrong size.  All work goes to torch's current
+stream of the plan's device; nothing synchronises the host.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+COMPS = "uvw"
+
+
+def field_shapes(nx, ny, nz):
+    """Ghost-shaped staggered array shapes (solver.py conventions)."""
+    return {
+        "u": (nx + 1, ny + 2, nz + 2),
+        "v": (nx + 2, ny + 1, nz + 2),
+        "w": (nx + 2, ny + 2, nz + 1),
+        "p": (nx + 2, ny + 2, nz + 2),
+    }
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(None)
+
+
+class HipPlan:
+    """One native plan: grid tables, Poisson tables, rocFFT plans, work buffers."""
+
+    def __init__(self, nx, ny, nz, Lx, Ly, Lz, z_f, z_c, gamma, stretching_type="symmetric",
+                 order=6, traj_order=2, n_traj_iters=2, top_wall_bc_type="dirichlet", nu=0.0,
+                 tanh_g=None, device=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("the HIP layer needs a ROCm GPU (torch.cuda.is_available() is False)")
+        lib = _lib.load()
+        self.device = torch.device("cuda", torch.cuda.current_device()) if device is None \
+            else torch.device(device)
+        if self.device.type != "cuda":
+            raise ValueError(f"HipPlan needs a cuda (HIP) device, got {self.device}")
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.Lx, self.Ly, self.Lz = float(Lx), float(Ly), float(Lz)
+        self.order = int(order)
+        self.nu = float(nu)
+        self.shapes = field_shapes(self.nx, self.ny, self.nz)
+        zf = np.ascontiguousarray(torch.as_tensor(z_f).detach().cpu().numpy(), dtype=np.float64)
+        zc = np.ascontiguousarray(torch.as_tensor(z_c).detach().cpu().numpy(), dtype=np.float64)
+        if zf.shape != (self.nz + 1,) or zc.shape != (self.nz + 2,):
+            raise ValueError(f"z_f must have nz+1 and z_c nz+2 entries, got {zf.shape} {zc.shape}")
+        if tanh_g is None:
+            # semilag.py:240-242 evaluates tanh(gamma) as a float64 torch scalar
+            tanh_g = torch.tanh(torch.tensor(float(gamma), dtype=torch.float64)).item()
+        self.tanh_g = float(tanh_g)
+        self._zf, self._zc = zf, zc  # keep alive for the create call
+        desc = _lib.GridDesc(
+            self.nx, self.ny, self.nz, self.Lx, self.Ly, self.Lz, float(gamma), self.tanh_g,
+            0 if stretching_type == "symmetric" else 1, self.order, int(traj_order),
+            int(n_traj_iters), 0 if top_wall_bc_type == "dirichlet" else 1, self.nu,
+            zf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            zc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+        )
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(lib.slc_plan_create(ctypes.byref(desc), self.device.index or 0,
+                                           ctypes.byref(handle)), "slc_plan_create")
+        self._h = handle
+        self._lib = lib
+        eps_z = 1e-12 * self.Lz
+        self.z_lo = float(zf[0] + eps_z)
+        self.z_hi = float(zf[-1] - eps_z)
+
+    # -- housekeeping -------------------------------------------------------
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.slc_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def workspace_bytes(self):
+        out = ctypes.c_int64()
+        _lib.check(self._lib.slc_plan_workspace_bytes(self._h, ctypes.byref(out)),
+                   "slc_plan_workspace_bytes")
+        return out.value
+
+    def profile(self, on=True):
+        """Bracket every labelled launch with hipEvents (see profile_report)."""
+        _lib.check(self._lib.slc_profile_enable(self._h, 1 if on else 0), "slc_profile_enable")
+
+    def profile_report(self):
+        """{label: (launches, total_ms)} since the last report (synchronises)."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        _lib.check(self._lib.slc_profile_report(self._h, buf, len(buf)), "slc_profile_report")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            label, n, ms = line.split()
+            out[label] = (int(n), float(ms))
+        return out
+
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _chk(self, t, shape, name, dtype=torch.float64):
+        if not isinstance(t, torch.Tensor):
+            raise TypeError(f"{name}: expected a torch.Tensor")
+        if t.device != self.device:
+            raise ValueError(f"{name}: on {t.device}, plan is on {self.device}")
+        if t.dtype != dtype:
+            raise TypeError(f"{name}: dtype {t.dtype}, expected {dtype}")
+        if not t.is_contiguous():
+            raise ValueError(f"{name}: must be contiguous")
+        if tuple(t.shape) != tuple(shape):
+            raise ValueError(f"{name}: shape {tuple(t.shape)}, expected {tuple(shape)}")
+        return t
+
+    def _counter(self, nclamp):
+        if nclamp is None:
+            return ctypes.c_void_p(None)
+        self._chk(nclamp, tuple(nclamp.shape), "nclamp", torch.int64)
+        if nclamp.numel() < 1:
+            raise ValueError("nclamp: needs one element")
+        return _ptr(nclamp)
+
+    def _uvw(self, u, v, w, tag=""):
+        s = self.shapes
+        return (self._chk(u, s["u"], "u" + tag), self._chk(v, s["v"], "v" + tag),
+                self._chk(w, s["w"], "w" + tag))
+
+    def n_arrival(self, comp):
+        return self.nx * self.ny * (self.nz - 1 if comp == "w" else self.nz)
+
+    def arrival_shape(self, comp):
+        return (self.nx, self.ny, self.nz - 1 if comp == "w" else self.nz)
+
+    # -- operators ----------------------------------------------------------
+
+    def apply_bc(self, u=None, v=None, w=None):
+        s = self.shapes
+        for t, c in ((u, "u"), (v, "v"), (w, "w")):
+            if t is not None:
+                self._chk(t, s[c], c)
+        _lib.check(self._lib.slc_apply_bc(self._h, _ptr(u), _ptr(v), _ptr(w), self._stream()),
+                   "slc_apply_bc")
+
+    def departure(self, comp, um, vm, wm, dt, xd, yd, zd):
+        self._uvw(um, vm, wm, "_mid")
+        n = self.n_arrival(comp)
+        for t, nm in ((xd, "xd"), (yd, "yd"), (zd, "zd")):
+            self._chk(t, (n,), nm)
+        _lib.check(self._lib.slc_departure(self._h, COMPS.index(comp), _ptr(um), _ptr(vm),
+                                           _ptr(wm), float(dt), _ptr(xd), _ptr(yd), _ptr(zd),
+                                           self._stream()), "slc_departure")
+
+    def gather(self, comp, field, xd, yd, zd, out, nclamp=None, layout=0):
+        if layout == 0:
+            self._chk(field, self.shapes[comp], "field")
+        else:
+            NZ = self.nz + 1 if comp == "w" else self.nz + 2
+            self._chk(field, (self.nx, self.ny, NZ), "node buffer")
+        n = self.n_arrival(comp)
+        for t, nm in ((xd, "xd"), (yd, "yd"), (zd, "zd"), (out, "out")):
+            self._chk(t, (n,), nm)
+        _lib.check(self._lib.slc_gather(self._h, COMPS.index(comp), _ptr(field), int(layout),
+                                        _ptr(xd), _ptr(yd), _ptr(zd), _ptr(out),
+                                        self._counter(nclamp), self._stream()), "slc_gather")
+
+    def stencil(self, comp, xd, yd, zd, ix, iy, k0, nclamp=None):
+        n = self.n_arrival(comp)
+        for t, nm in ((xd, "xd"), (yd, "yd"), (zd, "zd")):
+            self._chk(t, (n,), nm)
+        for t, nm in ((ix, "ix"), (iy, "iy"), (k0, "k0")):
+            self._chk(t, (n,), nm, torch.int32)
+        _lib.check(self._lib.slc_stencil(self._h, COMPS.index(comp), _ptr(xd), _ptr(yd),
+                                         _ptr(zd), _ptr(ix), _ptr(iy), _ptr(k0),
+                                         self._counter(nclamp), self._stream()), "slc_stencil")
+
+    def advect(self, u, v, w, um, vm, wm, dt, ustar, vstar, wstar, nclamp=None):
+        self._uvw(u, v, w)
+        self._uvw(um, vm, wm, "_mid")
+        self._uvw(ustar, vstar, wstar, "star")
+        _lib.check(self._lib.slc_advect(self._h, _ptr(u), _ptr(v), _ptr(w), _ptr(um), _ptr(vm),
+                                        _ptr(wm), float(dt), _ptr(ustar), _ptr(vstar),
+                                        _ptr(wstar), self._counter(nclamp), self._stream()),
+                   "slc_advect")
+
+    def bdf1_predict(self, u, v, w, dt, ustar, vstar, wstar, nclamp=None):
+        self._uvw(u, v, w)
+        self._uvw(ustar, vstar, wstar, "star")
+        _lib.check(self._lib.slc_bdf1_predict(self._h, _ptr(u), _ptr(v), _ptr(w), float(dt),
+                                              _ptr(ustar), _ptr(vstar), _ptr(wstar),
+                                              self._counter(nclamp), self._stream()),
+                   "slc_bdf1_predict")
+
+    def bdf2_predict(self, u, v, w, unm1, vnm1, wnm1, dt, ustar, vstar, wstar, nclamp=None):
+        self._uvw(u, v, w)
+        self._uvw(unm1, vnm1, wnm1, "_nm1")
+        self._uvw(ustar, vstar, wstar, "star")
+        _lib.check(self._lib.slc_bdf2_predict(self._h, _ptr(u), _ptr(v), _ptr(w), _ptr(unm1),
+                                              _ptr(vnm1), _ptr(wnm1), float(dt), _ptr(ustar),
+                                              _ptr(vstar), _ptr(wstar), self._counter(nclamp),
+                                              self._stream()), "slc_bdf2_predict")
+
+    def zsolve(self, comp, field, dt, theta=1.0):
+        self._chk(field, self.shapes[comp], comp)
+        _lib.check(self._lib.slc_zsolve(self._h, COMPS.index(comp), _ptr(field), float(dt),
+                                        float(theta), self._stream()), "slc_zsolve")
+
+    def divergence(self, u, v, w, out):
+        self._uvw(u, v, w)
+        self._chk(out, (self.nx, self.ny, self.nz), "div")
+        _lib.check(self._lib.slc_divergence(self._h, _ptr(u), _ptr(v), _ptr(w), _ptr(out),
+                                            self._stream()), "slc_divergence")
+
+    def poisson(self, div, p, rhs_scale=1.0):
+        self._chk(div, (self.nx, self.ny, self.nz), "div")
+        self._chk(p, self.shapes["p"], "p")
+        _lib.check(self._lib.slc_poisson(self._h, _ptr(div), float(rhs_scale), _ptr(p),
+                                         self._stream()), "slc_poisson")
+
+    def project(self, u, v, w, p, dt):
+        self._uvw(u, v, w)
+        self._chk(p, self.shapes["p"], "p")
+        _lib.check(self._lib.slc_project(self._h, _ptr(u), _ptr(v), _ptr(w), _ptr(p), float(dt),
+                                         self._stream()), "slc_project")
+
+    def project_step(self, u, v, w, rhs_scale, dt_proj, p=None):
+        self._uvw(u, v, w)
+        if p is not None:
+            self._chk(p, self.shapes["p"], "p")
+        _lib.check(self._lib.slc_project_step(self._h, _ptr(u), _ptr(v), _ptr(w),
+                                              float(rhs_scale), float(dt_proj), _ptr(p),
+                                              self._stream()), "slc_project_step")
+
+    def bulk_velocity(self, u, out):
+        self._chk(u, self.shapes["u"], "u")
+        self._chk(out, (4,), "out")
+        _lib.check(self._lib.slc_bulk_velocity(self._h, _ptr(u), _ptr(out), self._stream()),
+                   "slc_bulk_velocity")
+
+    def bulk_force(self, u, v, w, U_bulk, dt, out):
+        self._uvw(u, v, w)
+        self._chk(out, (4,), "out")
+        _lib.check(self._lib.slc_bulk_force(self._h, _ptr(u), _ptr(v), _ptr(w), float(U_bulk),
+                                            float(dt), _ptr(out), self._stream()),
+                   "slc_bulk_force")

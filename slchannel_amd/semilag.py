@@ -1,0 +1,160 @@
+"""Semi-Lagrangian advection on MI355X: drop-in SLAdvector + HipSL launch wrapper.
+
+``SLAdvector`` keeps the reference's constructor and ``advect`` contract
+(slchannel/semilag.py:130-671): same arguments and validation errors, same
+ghost-shaped persistent outputs (interiors written, ghosts left alone), the
+same ``n_clamped_last`` diagnostic as a 0-D int64 device tensor.  The work
+is done by libslc_hip.so: departure feet by the iterated midpoint rule with
+trilinear trajectory sampling and the analytic tanh inverse, then the
+order-4/6 tensor-product Lagrange gather, all in float64.
+
+``HipSL`` mirrors the reference's plugin object ``TritonSL``
+(slchannel/semilag_triton.py:347-455): ``departure(comp, dt) -> n`` and
+``gather(comp, buf, n) -> Tensor``, reading the trajectory velocity set by the
+last ``advect`` (or ``set_mid``) call.  It is what a maintainer binds in place
+of ``TritonSL``.
+
+There is no CPU or eager fallback: constructing either class without a ROCm
+device or without the built library raises.
+"""
+
+import torch
+
+from . import env
+from .plan import HipPlan
+
+
+class HipSL:
+    """Launch wrapper bound to one SLAdvector (TritonSL's method shape)."""
+
+    def __init__(self, adv):
+        self.adv = adv
+        plan = adv.plan
+        dev = plan.device
+        n_max = plan.nx * plan.ny * plan.nz
+        self.xd = torch.empty(n_max, dtype=torch.float64, device=dev)
+        self.yd = torch.empty(n_max, dtype=torch.float64, device=dev)
+        self.zd = torch.empty(n_max, dtype=torch.float64, device=dev)
+        self.nclamp = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.out = {c: torch.empty(plan.arrival_shape(c), dtype=torch.float64, device=dev)
+                    for c in "uvw"}
+        self._mid = None
+
+    def set_mid(self, u_mid, v_mid, w_mid):
+        """Trajectory velocity (ghost-shaped, BC-consistent) used by departure()."""
+        self._mid = (u_mid, v_mid, w_mid)
+
+    def departure(self, comp, dt):
+        """Feet of component `comp` for depth dt into self.xd/yd/zd; returns n."""
+        if self._mid is None:
+            raise RuntimeError("HipSL.departure: no trajectory velocity set (call set_mid)")
+        plan = self.adv.plan
+        n = plan.n_arrival(comp)
+        plan.departure(comp, *self._mid, float(dt), self.xd[:n], self.yd[:n], self.zd[:n])
+        return n
+
+    def gather(self, comp, buf, n):
+        """Interpolate `buf` (ghost-shaped field or (nx, ny, NZ) node buffer) at
+        the last feet; counts clamped feet into self.nclamp.  Returns the
+        reused (arrival-shaped) output buffer of `comp`."""
+        plan = self.adv.plan
+        layout = 0 if tuple(buf.shape) == plan.shapes[comp] else 1
+        out = self.out[comp].reshape(-1)
+        plan.gather(comp, buf, self.xd[:n], self.yd[:n], self.zd[:n], out, self.nclamp,
+                    layout=layout)
+        return self.out[comp]
+
+
+class SLAdvector:
+    """Semi-Lagrangian advection of the three staggered velocity components
+    (constructor and advect() as slchannel/semilag.py:133-202, 584-602)."""
+
+    def __init__(self, nx, ny, nz, dx, dy, Lx, Ly, Lz, z_f, z_c, gamma,
+                 stretching_type="symmetric", order=4, traj_order=2, n_traj_iters=2,
+                 top_wall_bc_type="dirichlet", interp_dtype="fp64", device=None, plan=None):
+        if order not in (4, 6):
+            raise ValueError(f"sl.interp_order must be 4 (tricubic) or 6 (triquintic), got {order}")
+        if traj_order not in (2, 4):
+            raise ValueError(
+                f"sl.traj_interp_order must be 2 (trilinear) or 4 (tricubic), got {traj_order}")
+        if stretching_type not in ("symmetric", "bottom"):
+            raise ValueError(
+                f"SLAdvector supports 'symmetric'/'bottom' stretching, got '{stretching_type}'")
+        if interp_dtype not in ("fp64", "fp32_accum64"):
+            raise ValueError(
+                f"sl.interp_dtype must be 'fp64' or 'fp32_accum64', got {interp_dtype}")
+        if nz + 1 < order:
+            raise ValueError(f"nz={nz} too small for interpolation order {order}")
+        if traj_order != 2:
+            raise ValueError("the HIP layer implements trilinear trajectories (traj_order=2)")
+        if not env.USE_HIP:
+            raise RuntimeError("SLCHANNEL_HIP=0: the HIP semi-Lagrangian layer is disabled")
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.dx, self.dy = dx, dy
+        self.Lx, self.Ly, self.Lz = Lx, Ly, Lz
+        self.gamma = float(gamma)
+        self.stretching_type = stretching_type
+        self.order = order
+        self.traj_order = traj_order
+        self.n_traj_iters = n_traj_iters
+        self.top_wall_bc_type = top_wall_bc_type
+        self.interp_dtype = interp_dtype
+        if plan is None:
+            plan = HipPlan(nx, ny, nz, Lx, Ly, Lz, z_f, z_c, gamma,
+                           stretching_type=stretching_type, order=order, traj_order=traj_order,
+                           n_traj_iters=n_traj_iters, top_wall_bc_type=top_wall_bc_type,
+                           device=device)
+        self.plan = plan
+        self.device = plan.device
+        self.z_lo, self.z_hi = plan.z_lo, plan.z_hi
+        self._tanh_g = plan.tanh_g
+        dev = self.device
+        self.ustar = torch.zeros(nx + 1, ny + 2, nz + 2, dtype=torch.float64, device=dev)
+        self.vstar = torch.zeros(nx + 2, ny + 1, nz + 2, dtype=torch.float64, device=dev)
+        self.wstar = torch.zeros(nx + 2, ny + 2, nz + 1, dtype=torch.float64, device=dev)
+        self._hip = HipSL(self)
+        self.n_clamped_last = torch.zeros((), dtype=torch.int64, device=dev)
+
+    def departure_coords(self, comp, dt_t, u_mid, v_mid, w_mid):
+        """Feet (x, y unwrapped; z not clamped) of `comp`, each (nx, ny, nzi)."""
+        self._hip.set_mid(u_mid, v_mid, w_mid)
+        n = self._hip.departure(comp, float(dt_t))
+        shape = self.plan.arrival_shape(comp)
+        return tuple(t[:n].clone().reshape(shape) for t in (self._hip.xd, self._hip.yd,
+                                                             self._hip.zd))
+
+    def advect(self, u, v, w, u_mid, v_mid, w_mid, dt_t, extra_rhs=None):
+        """semilag.py:584-671.  Returns (ustar, vstar, wstar) -- persistent
+        ghost-shaped buffers with interiors set (ghosts not refreshed); with
+        extra_rhs, a 4th element: list of interior-shaped triples interpolated
+        at the same feet."""
+        dt = float(dt_t)
+        plan = self.plan
+        hip = self._hip
+        hip.nclamp.zero_()
+        if extra_rhs is None:
+            plan.advect(u, v, w, u_mid, v_mid, w_mid, dt, self.ustar, self.vstar, self.wstar,
+                        hip.nclamp)
+            self.n_clamped_last = hip.nclamp[0].clone()
+            return self.ustar, self.vstar, self.wstar
+        hip.set_mid(u_mid, v_mid, w_mid)
+        fields = {"u": u, "v": v, "w": w}
+        outs = {"u": self.ustar, "v": self.vstar, "w": self.wstar}
+        extras = [dict() for _ in extra_rhs]
+        nx, ny, nz = self.nx, self.ny, self.nz
+        for ic, comp in enumerate("uvw"):
+            n = hip.departure(comp, dt)
+            shape = plan.arrival_shape(comp)
+            for t, triple in enumerate(extra_rhs):
+                # gather() does not count clamps for the extras (same feet)
+                out = torch.empty(n, dtype=torch.float64, device=self.device)
+                plan.gather(comp, triple[ic], hip.xd[:n], hip.yd[:n], hip.zd[:n], out, None)
+                extras[t][comp] = out.reshape(shape)
+            vals = hip.gather(comp, fields[comp], n)
+            if comp == "w":
+                outs[comp][1:nx + 1, 1:ny + 1, 1:nz] = vals
+            else:
+                outs[comp][1:nx + 1, 1:ny + 1, 1:nz + 1] = vals
+        self.n_clamped_last = hip.nclamp[0].clone()
+        return (self.ustar, self.vstar, self.wstar,
+                [(e["u"], e["v"], e["w"]) for e in extras])

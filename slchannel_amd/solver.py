@@ -1,0 +1,244 @@
+"""SLChannelFlow on MI355X: the SL-BDF2 step and the projection around it.
+
+Keeps the reference's step API -- ``step_sl_bdf2(dt) -> (u_bulk, forcing)``
+(slchannel/solver.py:813-1018) with the BDF1 bootstrap on the first step and
+on any dt change (solver.py:729-811, 834-844), ``apply_bc_uvw`` and the exact
+bulk-flux constraint (solver.py:694-723) -- and the YAML keys of the hot path
+(grid, domain, flow, boundary_conditions, time.dt, sl.*; docs/CONFIG.md).
+The run loop, statistics, checkpoints and plotting of the reference are not
+part of this layer.
+
+Per step the device does: ghost fill; the fused two-foot BDF2 predictor (far
+pass + near pass per component); ghost fill; three Thomas z-solves; ghost fill;
+divergence -> rocFFT -> per-mode Thomas -> inverse rocFFT -> gradient
+projection -> ghost fill; the deterministic bulk shift.  No host
+synchronisation: u_bulk and forcing come back as 0-D device tensors, as in the
+reference.
+"""
+
+import torch
+import yaml
+
+from . import env
+from .grid import generate_grid
+from .initflow import solenoidal_fields
+from .plan import HipPlan, field_shapes
+from .semilag import SLAdvector
+
+
+def _load_config(config_file=None, config=None):
+    if config is not None:
+        return config
+    with open(config_file) as f:
+        return yaml.safe_load(f)
+
+
+class SLChannelFlow:
+    """Channel-flow SL-BDF2 stepper on one HIP device.
+
+    ``config_file`` (YAML path) or ``config`` (dict) uses the reference's keys;
+    fields start as None until :meth:`set_fields` or :meth:`initialize`.
+    """
+
+    def __init__(self, config_file="config.yaml", config=None, device=None, z_tables=None):
+        cfg = _load_config(config_file, config)
+        self.nx = int(cfg["grid"]["nx"])
+        self.ny = int(cfg["grid"]["ny"])
+        self.nz = int(cfg["grid"]["nz"])
+        dom = cfg["domain"]
+        self.Lx, self.Ly, self.Lz = float(dom["Lx"]), float(dom["Ly"]), float(dom["Lz"])
+        self.stretching_type = dom.get("stretching_type", "symmetric")
+        if self.stretching_type not in ("symmetric", "bottom"):
+            raise ValueError(f"Invalid stretching type: {self.stretching_type}. "
+                             "slChannel supports 'symmetric' or 'bottom'")
+        flow = cfg["flow"]
+        self.nu = 1.0 / float(flow["Re"])
+        self.U_bulk = float(flow.get("U_bulk", 1.0))
+        self.gamma = float(flow["gamma"])
+        bc = cfg.get("boundary_conditions", {})
+        self.top_wall_bc_type = bc.get("top_wall", {}).get("type", "dirichlet")
+        if self.top_wall_bc_type not in ("dirichlet", "neumann"):
+            raise ValueError(f"Invalid top wall BC type: {self.top_wall_bc_type}")
+        self.dt = cfg.get("time", {}).get("dt", None)
+        sl = cfg.get("sl", {})
+        self.sl_order = int(sl.get("interp_order", 4))
+        self.sl_traj_order = int(sl.get("traj_interp_order", 2))
+        self.sl_traj_iters = int(sl.get("n_traj_iters", 2))
+        self.sl_interp_dtype = sl.get("interp_dtype", "fp64")
+        self.sl_bdf2_pressure = sl.get("bdf2_pressure", "noninc")
+        self.sl_bdf2_xy_rhs = sl.get("bdf2_xy_rhs", "extrap")
+        if self.sl_bdf2_pressure != "noninc" or self.sl_bdf2_xy_rhs != "extrap":
+            raise ValueError("the HIP layer implements bdf2_pressure='noninc' with "
+                             "bdf2_xy_rhs='extrap' (the production defaults)")
+
+        if device is None:
+            device = torch.device("cuda", torch.cuda.current_device()) \
+                if torch.cuda.is_available() else None
+        if device is None or torch.device(device).type != "cuda":
+            raise RuntimeError("SLChannelFlow (HIP layer) needs a ROCm GPU device")
+        self.device = torch.device(device)
+        if not env.USE_HIP:
+            raise RuntimeError("SLCHANNEL_HIP=0: the HIP layer is disabled")
+
+        if z_tables is None:
+            self.z_f, self.z_c, self.dz_f, self.dz_c = generate_grid(
+                self.gamma, self.nz, self.Lz, stretching_type=self.stretching_type)
+        else:
+            # caller-supplied tables (e.g. a grid read from a checkpoint)
+            self.z_f = torch.as_tensor(z_tables[0], dtype=torch.float64).cpu()
+            self.z_c = torch.as_tensor(z_tables[1], dtype=torch.float64).cpu()
+            self.dz_f = self.z_f[1:] - self.z_f[:-1]
+            self.dz_c = self.z_c[1:] - self.z_c[:-1]
+        self.dx = self.Lx / self.nx
+        self.dy = self.Ly / self.ny
+        self.total_volume = self.Lx * self.Ly * self.Lz
+
+        self.plan = HipPlan(self.nx, self.ny, self.nz, self.Lx, self.Ly, self.Lz, self.z_f,
+                            self.z_c, self.gamma, stretching_type=self.stretching_type,
+                            order=self.sl_order, traj_order=self.sl_traj_order,
+                            n_traj_iters=self.sl_traj_iters,
+                            top_wall_bc_type=self.top_wall_bc_type, nu=self.nu,
+                            device=self.device)
+        self.sl = SLAdvector(self.nx, self.ny, self.nz, self.dx, self.dy, self.Lx, self.Ly,
+                             self.Lz, self.z_f, self.z_c, self.gamma,
+                             stretching_type=self.stretching_type, order=self.sl_order,
+                             traj_order=self.sl_traj_order, n_traj_iters=self.sl_traj_iters,
+                             top_wall_bc_type=self.top_wall_bc_type,
+                             interp_dtype=self.sl_interp_dtype, plan=self.plan)
+        self.shapes = field_shapes(self.nx, self.ny, self.nz)
+        self.u = self.v = self.w = self.p = None
+        self.u_nm1 = self.v_nm1 = self.w_nm1 = None
+        self._star = None
+        self._dt_prev_step = None
+        self.forcing = 0.0
+        self.current_step = 0
+        self.time = 0.0
+        self._nclamp = torch.zeros(1, dtype=torch.int64, device=self.device)
+        self._bulk = torch.zeros(4, dtype=torch.float64, device=self.device)
+
+    # ------------------------------------------------------------------
+    # fields
+    # ------------------------------------------------------------------
+
+    def _dev(self, a, shape):
+        t = torch.as_tensor(a, dtype=torch.float64)
+        if tuple(t.shape) != tuple(shape):
+            raise ValueError(f"field shape {tuple(t.shape)}, expected {tuple(shape)}")
+        return t.to(self.device).contiguous().clone()
+
+    def set_fields(self, u, v, w, p=None):
+        """Adopt ghost-shaped fields (copied to the device).  Resets histories,
+        so the next step is a BDF1 bootstrap (as a restart is)."""
+        s = self.shapes
+        self.u, self.v, self.w = self._dev(u, s["u"]), self._dev(v, s["v"]), self._dev(w, s["w"])
+        self.p = self._dev(p, s["p"]) if p is not None else torch.zeros(
+            s["p"], dtype=torch.float64, device=self.device)
+        self.u_nm1 = self.v_nm1 = self.w_nm1 = None
+        self._dt_prev_step = None
+
+    def initialize(self, seed, urms=0.1, kmin=1.0, kmax=8.0, spectrum="flat"):
+        """Seeded solenoidal Poiseuille + perturbation, boundary-filled and
+        projected with dt = 1 (solver.py:599-618)."""
+        u, v, w = solenoidal_fields(self.nx, self.ny, self.nz, self.Lx, self.Ly, self.Lz,
+                                    self.z_f.numpy(), self.z_c.numpy(), seed, urms=urms,
+                                    kmin=kmin, kmax=kmax, spectrum=spectrum, U_bulk=self.U_bulk)
+        self.set_fields(u, v, w)
+        self.apply_bc_uvw()
+        self.project_initial()
+
+    def project_initial(self):
+        """solver.py:599-618: one projection with dt = 1 and a ghost fill."""
+        self.plan.project_step(self.u, self.v, self.w, 1.0, 1.0, self.p)
+
+    def apply_bc_uvw(self):
+        """apply_bc_all (solver.py:70-107) on the current fields."""
+        self.plan.apply_bc(self.u, self.v, self.w)
+
+    def _ensure_star(self):
+        if self._star is None:
+            s = self.shapes
+            self._star = [torch.zeros(s[c], dtype=torch.float64, device=self.device)
+                          for c in "uvw"]
+
+    # ------------------------------------------------------------------
+    # step
+    # ------------------------------------------------------------------
+
+    def _apply_bulk_forcing(self, dt):
+        """solver.py:694-723: exact uniform shift of u onto U_bulk; forcing is
+        the correction / dt.  Returns 0-D device tensors."""
+        self.plan.bulk_force(self.u, self.v, self.w, self.U_bulk, dt, self._bulk)
+        self.forcing = self._bulk[1].clone()
+        return self._bulk[2].clone(), self.forcing
+
+    def _implicit_and_project(self, dt_z, rhs_scale, dt_proj):
+        plan = self.plan
+        plan.apply_bc(self.u, self.v, self.w)
+        plan.zsolve("u", self.u, dt_z, 1.0)
+        plan.zsolve("v", self.v, dt_z, 1.0)
+        plan.zsolve("w", self.w, dt_z, 1.0)
+        plan.apply_bc(self.u, self.v, self.w)
+        plan.project_step(self.u, self.v, self.w, rhs_scale, dt_proj, self.p)
+
+    def _step_sl_bdf1_bootstrap(self, dt):
+        """solver.py:729-811: U* = V^n, one foot at dt, theta = 1 at dt."""
+        self._ensure_star()
+        if self.u_nm1 is None:
+            self.u_nm1, self.v_nm1, self.w_nm1 = self.u.clone(), self.v.clone(), self.w.clone()
+        else:
+            self.u_nm1.copy_(self.u)
+            self.v_nm1.copy_(self.v)
+            self.w_nm1.copy_(self.w)
+        self._nclamp.zero_()
+        us, vs, ws = self._star
+        self.plan.bdf1_predict(self.u, self.v, self.w, dt, us, vs, ws, self._nclamp)
+        # adopt the predictor buffers; the old field buffers become scratch
+        self._star = [self.u, self.v, self.w]
+        self.u, self.v, self.w = us, vs, ws
+        self._implicit_and_project(dt, dt, dt)
+        self.sl.n_clamped_last = self._nclamp[0].clone()
+        self._dt_prev_step = dt
+        return self._apply_bulk_forcing(dt)
+
+    def step_sl_bdf2(self, dt):
+        """solver.py:813-1018 (noninc pressure, extrapolated xy RHS)."""
+        dt = float(dt)
+        self.apply_bc_uvw()
+        if (self.u_nm1 is None or self._dt_prev_step is None
+                or abs(dt - self._dt_prev_step) > 1e-12 * dt):
+            if self._dt_prev_step is not None and self.u_nm1 is not None:
+                print(f"[bdf2] dt changed ({self._dt_prev_step:.6g} -> {dt:.6g}): "
+                      "BDF1 re-bootstrap", flush=True)
+            out = self._step_sl_bdf1_bootstrap(dt)
+        else:
+            self._ensure_star()
+            self._nclamp.zero_()
+            us, vs, ws = self._star
+            self.plan.bdf2_predict(self.u, self.v, self.w, self.u_nm1, self.v_nm1, self.w_nm1,
+                                   dt, us, vs, ws, self._nclamp)
+            # rotate: V^{n-1} <- V^n, V <- predictor, scratch <- old V^{n-1}
+            self._star = [self.u_nm1, self.v_nm1, self.w_nm1]
+            self.u_nm1, self.v_nm1, self.w_nm1 = self.u, self.v, self.w
+            self.u, self.v, self.w = us, vs, ws
+            dt_eff = (2.0 / 3.0) * dt
+            self._implicit_and_project(dt_eff, dt_eff, dt_eff)
+            self.sl.n_clamped_last = self._nclamp[0].clone()
+            self._dt_prev_step = dt
+            out = self._apply_bulk_forcing(dt)
+        self.current_step += 1
+        self.time += dt
+        return out
+
+    # ------------------------------------------------------------------
+    # diagnostics (device-side, no host sync unless the caller reads them)
+    # ------------------------------------------------------------------
+
+    def divergence(self):
+        out = torch.empty(self.nx, self.ny, self.nz, dtype=torch.float64, device=self.device)
+        self.plan.divergence(self.u, self.v, self.w, out)
+        return out
+
+    def bulk_velocity(self):
+        out = torch.zeros(4, dtype=torch.float64, device=self.device)
+        self.plan.bulk_velocity(self.u, out)
+        return out[3]

@@ -1,0 +1,320 @@
+"""Parity of the HIP layer (libslc_hip.so) against the reference's golden
+vectors and against the CPU oracle, on identical inputs.
+
+Bars (stated per check): feet bit-exact except z-face ties; stencil indices
+bit-exact; ghost fill, divergence and projection bit-exact; advected fields
+within 1e-13 of the max; solves and full steps within 1e-12 relative
+(Thomas vs PCR, rocFFT vs pocketfft rounding); max|div| < 1e-11 and the bulk
+velocity pinned at 1.0 after every step.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+EPS = np.finfo(float).eps
+
+
+def rel_err(a, b):
+    a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    b = b.detach().cpu().numpy() if isinstance(b, torch.Tensor) else np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def cuda(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a)).to(dtype).cuda()
+
+
+def make_plan(g, order=None, nu=0.0, top=None):
+    from slchannel_amd import HipPlan
+
+    nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
+    return HipPlan(nx, ny, nz, float(g["Lx"]), float(g["Ly"]), float(g["Lz"]), g["z_f"],
+                   g["z_c"], float(g["gamma"]),
+                   stretching_type=str(g.get("stretching", "symmetric")),
+                   order=int(order if order is not None else g["order"]),
+                   top_wall_bc_type=top or str(g.get("top", "dirichlet")), nu=nu)
+
+
+def feet_ok(hip, ref, Lx):
+    same = np.all([(h == r) for h, r in zip(hip, ref)], axis=0)
+    dev = max(float(np.abs(h - r).max()) for h, r in zip(hip, ref))
+    return same.mean() >= 0.995 and dev <= 4 * EPS * Lx, same, dev
+
+
+# ---------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("top", ["dirichlet", "neumann"])
+def test_ghost_fill_bitexact(check, top):
+    from slchannel_amd import HipPlan
+    from slchannel_amd.grid import generate_grid
+
+    nx, ny, nz = 13, 11, 17
+    z_f, z_c, _, _ = generate_grid(1.5, nz, 2.0)
+    plan = HipPlan(nx, ny, nz, 4.0, 3.0, 2.0, z_f, z_c, 1.5, order=4, top_wall_bc_type=top)
+    rng = np.random.default_rng(0)
+    u, v, w = (rng.normal(size=s) for s in ((nx + 1, ny + 2, nz + 2), (nx + 2, ny + 1, nz + 2),
+                                             (nx + 2, ny + 2, nz + 1)))
+    tu, tv, tw = cuda(u), cuda(v), cuda(w)
+    plan.apply_bc(tu, tv, tw)
+    orc.apply_bc_all(u, v, w, top)
+    for nm, a, b in (("u", tu, u), ("v", tv, v), ("w", tw, w)):
+        check(f"bc {nm} ({top})", np.array_equal(a.cpu().numpy(), b), "bit-exact")
+
+
+@pytest.mark.parametrize("name", ["o4", "o6", "o6_bottom", "o4_clamp"])
+def test_feet_stencils_advect_vs_reference(check, golden, name):
+    g = golden("advect_" + name)
+    plan = make_plan(g)
+    Lx = float(g["Lx"])
+    dt = float(g["dt"])
+    um, vm, wm = cuda(g["um"]), cuda(g["vm"]), cuda(g["wm"])
+    for comp in "uvw":
+        n = plan.n_arrival(comp)
+        xd, yd, zd = (torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(3))
+        plan.departure(comp, um, vm, wm, dt, xd, yd, zd)
+        ok, same, dev = feet_ok([t.cpu().numpy() for t in (xd, yd, zd)],
+                                [g[f"{c}d_{comp}"] for c in "xyz"], Lx)
+        check(f"feet {comp} vs reference", ok,
+              f"identical={int(same.sum())}/{same.size} max|d|={dev:.1e}")
+        ix, iy, k0 = (torch.empty(n, dtype=torch.int32, device="cuda") for _ in range(3))
+        gx, gy, gz = (cuda(g[f"{c}d_{comp}"]) for c in "xyz")
+        plan.stencil(comp, gx, gy, gz, ix, iy, k0)
+        same_idx = ((ix.cpu().numpy() == g[f"ix_{comp}"]) & (iy.cpu().numpy() == g[f"iy_{comp}"])
+                    & (k0.cpu().numpy() == g[f"k0_{comp}"]))
+        check(f"stencil indices {comp} bit-exact", same_idx.all(),
+              f"identical={int(same_idx.sum())}/{same_idx.size}")
+    from slchannel_amd import SLAdvector
+
+    nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
+    adv = SLAdvector(nx, ny, nz, Lx / nx, float(g["Ly"]) / ny, Lx, float(g["Ly"]),
+                     float(g["Lz"]), g["z_f"], g["z_c"], float(g["gamma"]),
+                     stretching_type=str(g["stretching"]), order=int(g["order"]), plan=plan)
+    us, vs, ws = adv.advect(cuda(g["u"]), cuda(g["v"]), cuda(g["w"]), um, vm, wm, dt)
+    for nm, a, b in (("u", us, g["ustar"]), ("v", vs, g["vstar"]), ("w", ws, g["wstar"])):
+        e = rel_err(a, b)
+        check(f"advect {nm} vs reference", e < 1e-13, f"rel={e:.1e}")
+    ncl = int(adv.n_clamped_last.item())
+    check("n_clamped", ncl == int(g["n_clamped"]), f"{ncl} vs {int(g['n_clamped'])}")
+
+
+def _random_case(nx, ny, nz, order, stretching, seed, gamma=1.6, cfl=2.5):
+    from slchannel_amd.grid import generate_grid
+
+    Lx, Ly, Lz = 4 * math.pi, 2 * math.pi, 2.0
+    z_f, z_c, _, _ = (t.numpy() for t in generate_grid(gamma, nz, Lz, stretching_type=stretching))
+    rng = np.random.default_rng(seed)
+    shapes = ((nx + 1, ny + 2, nz + 2), (nx + 2, ny + 1, nz + 2), (nx + 2, ny + 2, nz + 1))
+    # smooth fields from a few random modes plus noise-free ghosts
+    fields = []
+    for comp, s in zip("uvw", shapes):
+        I = np.arange(s[0])[:, None, None]
+        J = np.arange(s[1])[None, :, None]
+        zz = (z_c if comp != "w" else z_f)[None, None, :]
+        f = np.zeros(s)
+        for _ in range(4):
+            a = rng.normal()
+            f += a * np.sin(2 * math.pi * rng.integers(1, 3) * I / nx + rng.uniform(0, 6)) * \
+                np.cos(2 * math.pi * rng.integers(0, 3) * J / ny + rng.uniform(0, 6)) * \
+                np.cos(math.pi * rng.uniform(0.5, 2) * zz / Lz + rng.uniform(0, 6))
+        fields.append(f)
+    mids = [1.0 + 0.3 * fields[0], 0.3 * fields[1], 0.2 * fields[2]]
+    for arr in (fields, mids):
+        orc.apply_bc_all(*arr)
+    dx = Lx / nx
+    dt = cfl * dx / float(np.abs(mids[0]).max())
+    return dict(nx=nx, ny=ny, nz=nz, Lx=Lx, Ly=Ly, Lz=Lz, gamma=gamma, order=order,
+                stretching=stretching, z_f=z_f, z_c=z_c, dt=dt, fields=fields, mids=mids)
+
+
+@pytest.mark.parametrize("order,stretching", [(4, "symmetric"), (6, "symmetric"),
+                                              (6, "bottom")])
+def test_advect_vs_oracle(check, order, stretching):
+    c = _random_case(40, 36, 48, order, stretching, seed=order + len(stretching))
+    plan = make_plan(c)
+    sl = orc.SLOracle(c["nx"], c["ny"], c["nz"], c["Lx"] / c["nx"], c["Ly"] / c["ny"], c["Lx"],
+                      c["Ly"], c["Lz"], c["z_f"], c["z_c"], c["gamma"],
+                      stretching_type=stretching, order=order)
+    sl.set_tanh_g(plan.tanh_g)
+    mids = [cuda(a) for a in c["mids"]]
+    dt = c["dt"]
+    for ic, comp in enumerate("uvw"):
+        n = plan.n_arrival(comp)
+        xd, yd, zd = (torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(3))
+        plan.departure(comp, *mids, dt, xd, yd, zd)
+        ref = sl.departure(comp, *c["mids"], dt)
+        ok, same, dev = feet_ok([t.cpu().numpy() for t in (xd, yd, zd)], ref, c["Lx"])
+        check(f"feet {comp} vs oracle", ok,
+              f"identical={int(same.sum())}/{same.size} max|d|={dev:.1e}")
+        ix, iy, k0 = (torch.empty(n, dtype=torch.int32, device="cuda") for _ in range(3))
+        gx, gy, gz = (cuda(a) for a in ref)
+        plan.stencil(comp, gx, gy, gz, ix, iy, k0)
+        vals, _, (rix, riy, rk0) = sl.gather(comp, c["fields"][ic], *ref, want_index=True)
+        same_idx = ((ix.cpu().numpy() == rix) & (iy.cpu().numpy() == riy)
+                    & (k0.cpu().numpy() == rk0))
+        check(f"stencil {comp} vs oracle", same_idx.all(),
+              f"identical={int(same_idx.sum())}/{same_idx.size}")
+        out = torch.empty(n, dtype=torch.float64, device="cuda")
+        plan.gather(comp, cuda(c["fields"][ic]), gx, gy, gz, out)
+        e = rel_err(out, vals)
+        check(f"gather {comp} vs oracle", e < 1e-13, f"rel={e:.1e}")
+    shapes = plan.shapes
+    outs = [torch.zeros(shapes[k], dtype=torch.float64, device="cuda") for k in "uvw"]
+    ncl = torch.zeros(1, dtype=torch.int64, device="cuda")
+    plan.advect(*[cuda(a) for a in c["fields"]], *mids, dt, *outs, ncl)
+    ref_out = sl.advect(*c["fields"], *c["mids"], dt)
+    for nm, a, b in zip("uvw", outs, ref_out):
+        e = rel_err(a, b)
+        check(f"advect {nm} vs oracle", e < 1e-13, f"rel={e:.1e}")
+    check("clamp count vs oracle", int(ncl.item()) == sl.n_clamped_last,
+          f"{int(ncl.item())} vs {sl.n_clamped_last}")
+
+
+@pytest.mark.parametrize("tag", ["sym", "bot"])
+def test_solves_vs_reference(check, golden, tag):
+    g = golden("solve_small")
+    nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
+    gg = dict(nx=nx, ny=ny, nz=nz, Lx=g["Lx"], Ly=g["Ly"], Lz=g["Lz"], gamma=g["gamma"],
+              z_f=g[f"{tag}_z_f"], z_c=g[f"{tag}_z_c"],
+              stretching="symmetric" if tag == "sym" else "bottom")
+    top = str(g[f"{tag}_top"])
+    plan = make_plan(gg, order=4, nu=float(g[f"{tag}_nu"]), top=top)
+    dt = float(g[f"{tag}_dt"])
+    for theta, th in ((1.0, "t1"), (0.5, "t05")):
+        for comp in "uvw":
+            f = cuda(g[f"{tag}_{comp}"])
+            plan.zsolve(comp, f, dt, theta)
+            e = rel_err(f, g[f"{tag}_{th}_{comp}"])
+            check(f"zsolve {comp} theta={theta}", e < 1e-13, f"rel={e:.1e}")
+    p = torch.zeros(plan.shapes["p"], dtype=torch.float64, device="cuda")
+    plan.poisson(cuda(g[f"{tag}_div"]), p, 1.0)
+    e = rel_err(p, g[f"{tag}_p"])
+    check("poisson p vs reference", e < 1e-12, f"rel={e:.1e}")
+
+
+def test_divergence_projection_bulk_vs_oracle(check):
+    c = _random_case(24, 20, 28, 4, "symmetric", seed=9)
+    plan = make_plan(c)
+    nx, ny, nz = c["nx"], c["ny"], c["nz"]
+    dx, dy = c["Lx"] / nx, c["Ly"] / ny
+    z_f, z_c = c["z_f"], c["z_c"]
+    dz_f, dz_c = z_f[1:] - z_f[:-1], z_c[1:] - z_c[:-1]
+    u, v, w = (a.copy() for a in c["fields"])
+    tu, tv, tw = cuda(u), cuda(v), cuda(w)
+    div = torch.empty(nx, ny, nz, dtype=torch.float64, device="cuda")
+    plan.divergence(tu, tv, tw, div)
+    ref = orc.compute_divergence(u, v, w, nx, ny, nz, dx, dy, dz_f)
+    check("divergence bit-exact", np.array_equal(div.cpu().numpy(), ref), "")
+    rng = np.random.default_rng(3)
+    p = rng.normal(size=(nx + 2, ny + 2, nz + 2))
+    plan.project(tu, tv, tw, cuda(p), 0.37)
+    orc.project_velocity(u, v, w, p, nx, ny, nz, dx, dy, dz_c, dz_f, 0.37)
+    for nm, a, b in (("u", tu, u), ("v", tv, v), ("w", tw, w)):
+        check(f"project {nm} bit-exact", np.array_equal(a.cpu().numpy(), b), "")
+    out = torch.zeros(4, dtype=torch.float64, device="cuda")
+    plan.bulk_velocity(tu, out)
+    cv = np.broadcast_to((dx * dy * dz_f).reshape(1, 1, -1), (nx, ny, nz))
+    ub = orc.compute_bulk_velocity(u, cv, c["Lx"] * c["Ly"] * c["Lz"])
+    e = abs(out[3].item() - ub) / abs(ub)
+    check("bulk velocity", e < 1e-14, f"rel={e:.1e}")
+    # fused projection step: div -> rocFFT -> Thomas -> iFFT -> grad -> ghosts
+    u2, v2, w2 = (a.copy() for a in c["fields"])
+    t2 = [cuda(a) for a in (u2, v2, w2)]
+    pp = torch.zeros(plan.shapes["p"], dtype=torch.float64, device="cuda")
+    plan.project_step(*t2, 0.5, 0.5, pp)
+    fft = orc.initialize_fft_solver(nx, ny, nz, dx, dy, dz_c, dz_f)
+    d2 = orc.compute_divergence(u2, v2, w2, nx, ny, nz, dx, dy, dz_f)
+    phi = orc.solve_poisson_fft(d2 / 0.5, fft)
+    orc.project_velocity(u2, v2, w2, phi, nx, ny, nz, dx, dy, dz_c, dz_f, 0.5)
+    orc.apply_bc_all(u2, v2, w2)
+    for nm, a, b in (("u", t2[0], u2), ("v", t2[1], v2), ("w", t2[2], w2), ("p", pp, phi)):
+        e = rel_err(a, b)
+        check(f"project_step {nm}", e < 1e-12, f"rel={e:.1e}")
+    d3 = torch.empty(nx, ny, nz, dtype=torch.float64, device="cuda")
+    plan.divergence(*t2, d3)
+    md = d3.abs().max().item()
+    check("post-projection max|div|", md < 1e-11, f"{md:.2e}")
+
+
+def _flow_from_golden(g):
+    from slchannel_amd import SLChannelFlow
+
+    cfg = {
+        "grid": {"nx": int(g["nx"]), "ny": int(g["ny"]), "nz": int(g["nz"])},
+        "domain": {"Lx": float(g["Lx"]), "Ly": float(g["Ly"]), "Lz": float(g["Lz"]),
+                   "stretching_type": str(g["stretching"])},
+        "flow": {"Re": float(g["Re"]), "U_bulk": 1.0, "gamma": float(g["gamma"])},
+        "boundary_conditions": {"top_wall": {"type": str(g["top"])}},
+        "time": {"dt": float(g["dt"])},
+        "sl": {"interp_order": int(g["order"]), "n_traj_iters": 2},
+    }
+    return SLChannelFlow(config=cfg, z_tables=(g["z_f"], g["z_c"]))
+
+
+@pytest.mark.parametrize("name", ["o6", "o4_bottom_neumann"])
+def test_step_vs_reference(check, golden, name):
+    g = golden("step_" + name)
+    flow = _flow_from_golden(g)
+    flow.set_fields(g["u0"], g["v0"], g["w0"], g["p0"])
+    dt = float(g["dt"])
+    for n in range(int(g["n_steps"])):
+        ub, fo = flow.step_sl_bdf2(dt)
+        check(f"step {n + 1} u_bulk", abs(ub.item() - g["u_bulk"][n]) < 1e-14,
+              f"{ub.item()!r}")
+        check(f"step {n + 1} forcing",
+              abs(fo.item() - g["forcing"][n]) < 1e-12 * max(1.0, abs(g["forcing"][n])),
+              f"{fo.item():.6e} vs {g['forcing'][n]:.6e}")
+        check(f"step {n + 1} n_clamped", int(flow.sl.n_clamped_last.item()) == g["n_clamped"][n],
+              "")
+        if f"u_{n + 1}" in g:
+            for nm, a in (("u", flow.u), ("v", flow.v), ("w", flow.w), ("p", flow.p)):
+                e = rel_err(a, g[f"{nm}_{n + 1}"])
+                check(f"step {n + 1} {nm}", e < 1e-12, f"rel={e:.1e}")
+
+
+def _config(n, order=6, Re=2800.0, nz=None):
+    return {
+        "grid": {"nx": n, "ny": n, "nz": nz or n},
+        "domain": {"Lx": 4 * math.pi, "Ly": 2 * math.pi, "Lz": 2.0},
+        "flow": {"Re": Re, "U_bulk": 1.0, "gamma": 1.5},
+        "sl": {"interp_order": order},
+    }
+
+
+@pytest.mark.parametrize("order", [4, 6])
+def test_step_vs_oracle_seeded(check, order):
+    """Seeded solenoidal init, BDF1 + BDF2 steps, HIP vs oracle every step."""
+    from slchannel_amd import SLChannelFlow
+
+    flow = SLChannelFlow(config=_config(32, order=order, nz=24))
+    flow.initialize(seed=1001, urms=0.1, kmax=6.0)
+    co = orc.ChannelOracle(flow.nx, flow.ny, flow.nz, flow.Lx, flow.Ly, flow.Lz, flow.gamma,
+                           2800.0, order=order, z_f=flow.z_f.numpy(), z_c=flow.z_c.numpy(),
+                           tanh_g=flow.plan.tanh_g)
+    co.set_fields(flow.u.cpu().numpy(), flow.v.cpu().numpy(), flow.w.cpu().numpy(),
+                  flow.p.cpu().numpy())
+    from slchannel_amd.diagnostics import cfl_dt
+
+    dt = cfl_dt(flow, 3.0)  # trajectory CFL 3 (operators.py:839-940)
+    worst, per_step = 0.0, []
+    for n in range(8):
+        ub, fo = flow.step_sl_bdf2(dt)
+        rub, rfo = co.step_sl_bdf2(dt)
+        e = max(rel_err(a, b) for a, b in ((flow.u, co.u), (flow.v, co.v), (flow.w, co.w)))
+        per_step.append(f"{e:.0e}")
+        worst = max(worst, e)
+        md = flow.divergence().abs().max().item()
+        check(f"step {n + 1} max|div|", md < 1e-11, f"{md:.2e}")
+        check(f"step {n + 1} u_bulk", abs(ub.item() - 1.0) < 1e-14, f"{ub.item()!r}")
+        check(f"step {n + 1} clamps vs oracle",
+              int(flow.sl.n_clamped_last.item()) == co.n_clamped_last,
+              f"{int(flow.sl.n_clamped_last.item())} vs {co.n_clamped_last}")
+    check("fields vs oracle over 8 steps", worst < 1e-12,
+          f"max rel={worst:.1e} per step={per_step} dt={dt:.4f}")
