@@ -107,6 +107,13 @@ int slc_gather(slc_plan* plan, int comp, const double* field, int layout, const 
 int slc_stencil(slc_plan* plan, int comp, const double* xd, const double* yd, const double* zd,
                 int32_t* ix, int32_t* iy, int32_t* k0, int64_t* nclamp, void* stream);
 
+/* z-locate of n points (device doubles, clamped into [z_lo, z_hi] first):
+ * node-interval index m0 of component comp's z nodes, by the analytic inverse
+ * tanh map (m0_exact) and by the table-refined fast path the kernels use
+ * (m0_fast).  Replaces SLAdvector._locate_z (semilag.py:413-420). */
+int slc_locate_z(slc_plan* plan, int comp, const double* z, int64_t n, int32_t* m0_exact,
+                 int32_t* m0_fast, void* stream);
+
 /* Full semi-Lagrangian advection of (u, v, w) along (um, vm, wm) over dt,
  * writing the interiors of the ghost-shaped outputs (ghosts and the w wall
  * faces untouched) and adding the clamp count to *nclamp (device int64,
@@ -169,6 +176,12 @@ int slc_bulk_velocity(slc_plan* plan, const double* u, double* out, void* stream
  * Replaces SLChannelFlow._apply_bulk_forcing (solver.py:694-723). */
 int slc_bulk_force(slc_plan* plan, double* u, double* v, double* w, double U_bulk, double dt,
                    double* out, void* stream);
+
+/* Tuning options (no counterpart in the reference).
+ * SLC_OPT_GATHER_TILE: -1 = direct global-load gathers; 0..3 = LDS tile shape
+ * (0: 4x8x32, 1: 2x8x32, 2: 2x4x64, 3: 4x4x32 arrival points). */
+#define SLC_OPT_GATHER_TILE 1
+int slc_set_option(slc_plan* plan, int option, int value);
 
 /* Opt-in launch timing (no counterpart in the reference, whose benchmarks
  * time whole steps with perf_counter + cuda.synchronize,

@@ -51,6 +51,8 @@ def _load():
     lib.orc_sample.restype = None
     lib.orc_interp.argtypes = [gp, ctypes.c_int, dp, ctypes.c_long, dp, dp, dp, dp]
     lib.orc_interp.restype = None
+    lib.orc_locate_z.argtypes = [gp, ctypes.c_int, ctypes.c_long, dp, ip]
+    lib.orc_locate_z.restype = None
     lib.orc_face_coord.argtypes = [gp, ctypes.c_double]
     lib.orc_face_coord.restype = ctypes.c_double
     _lib = lib
@@ -176,6 +178,14 @@ class SLOracle:
     def face_coord(self, z):
         lib = _load()
         return np.array([lib.orc_face_coord(ctypes.byref(self._g), float(v)) for v in z])
+
+    def locate_z(self, comp, z):
+        """semilag.py:413-420 (z clamped into [z_lo, z_hi] first)."""
+        lib = _load()
+        z = np.ascontiguousarray(z, dtype=np.float64).ravel()
+        out = np.empty(z.size, dtype=np.int32)
+        lib.orc_locate_z(ctypes.byref(self._g), "uvw".index(comp), z.size, _dptr(z), _iptr(out))
+        return out
 
     def departure(self, comp, um, vm, wm, dt):
         """semilag.py:534-552: feet (x, y unwrapped; z not yet clamped)."""

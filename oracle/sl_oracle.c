@@ -101,6 +101,13 @@ static long locate_z(const orc_grid* g, const orc_spec* s, double z) {
     return c + (z >= s->znodes[c + 1] ? 1 : 0);
 }
 
+/* _locate_z for n points of component comp (z clamped into [z_lo, z_hi]
+ * first, as every caller in the reference does). */
+void orc_locate_z(const orc_grid* g, int comp, long n, const double* z, int32_t* m0) {
+    orc_spec s = make_spec(g, comp);
+    for (long i = 0; i < n; ++i) m0[i] = (int32_t)locate_z(g, &s, clampd(z[i], g->z_lo, g->z_hi));
+}
+
 /* semilag.py:426-439 (_uniform_weights) with semilag.py:117-127 baked in */
 static void uniform_weights(double t, int P, double* w) {
     double d[8], inv[8];

@@ -14,9 +14,9 @@ LIB_PATH = os.path.join(HERE, "libslc_hip.so")
 EXPORTS = (
     "slc_version", "slc_last_error", "slc_plan_create", "slc_plan_destroy",
     "slc_plan_workspace_bytes", "slc_apply_bc", "slc_departure", "slc_gather",
-    "slc_stencil", "slc_advect", "slc_bdf1_predict", "slc_bdf2_predict", "slc_zsolve", "slc_divergence",
+    "slc_stencil", "slc_locate_z", "slc_advect", "slc_bdf1_predict", "slc_bdf2_predict", "slc_zsolve", "slc_divergence",
     "slc_poisson", "slc_project", "slc_project_step", "slc_bulk_velocity", "slc_bulk_force",
-    "slc_profile_enable", "slc_profile_report",
+    "slc_set_option", "slc_profile_enable", "slc_profile_report",
 )
 
 
@@ -60,6 +60,7 @@ def load():
         "slc_departure": ([vp, i, vp, vp, vp, d, vp, vp, vp, vp], i),
         "slc_gather": ([vp, i, vp, i, vp, vp, vp, vp, i64p, vp], i),
         "slc_stencil": ([vp, i, vp, vp, vp, vp, vp, vp, i64p, vp], i),
+        "slc_locate_z": ([vp, i, vp, ctypes.c_int64, vp, vp, vp], i),
         "slc_advect": ([vp, vp, vp, vp, vp, vp, vp, d, vp, vp, vp, i64p, vp], i),
         "slc_bdf1_predict": ([vp, vp, vp, vp, d, vp, vp, vp, i64p, vp], i),
         "slc_bdf2_predict": ([vp, vp, vp, vp, vp, vp, vp, d, vp, vp, vp, i64p, vp], i),
@@ -70,6 +71,7 @@ def load():
         "slc_project_step": ([vp, vp, vp, vp, d, d, vp, vp], i),
         "slc_bulk_velocity": ([vp, vp, vp, vp], i),
         "slc_bulk_force": ([vp, vp, vp, vp, d, d, vp, vp], i),
+        "slc_set_option": ([vp, i, i], i),
         "slc_profile_enable": ([vp, i], i),
         "slc_profile_report": ([vp, ctypes.c_char_p, ctypes.c_int64], i),
     }

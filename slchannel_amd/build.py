@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libslc_hip.so")
-SOURCES = ["slc_api.hip", "slc_sl.hip", "slc_ops.hip"]
+SOURCES = ["slc_api.hip", "slc_sl.hip", "slc_ops.hip", "slc_tiled.hip", "slc_step.hip"]
 ARCH = os.environ.get("SLC_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -78,11 +78,17 @@ struct slc_plan {
     double* planes = nullptr;   // (nz, nx, ny) real
     double* dhat = nullptr;     // (nz, nx, nky) complex
     double* cpw = nullptr;      // (nz, nx, nky) Thomas c'
-    double* ztab = nullptr;     // 3 * (nz + 2) z-solve factor
+    double* ztab = nullptr;     // per component: 3 * (nz + 2) z-solve factor
+    double zkey[3][2] = {};     // (dt, theta) the factor of each component was built for
+    bool zvalid[3] = {false, false, false};
     double* partials = nullptr;
     int max_partials = 0;
+    double* us[3] = {nullptr, nullptr, nullptr};  // U* = 2V^n - V^{n-1} (ghost-shaped)
+    double* nfoot = nullptr;                      // near feet of one component (3 x N)
+    ArrivalTabs at[3] = {};
     // opt-in launch timing: hipEvents recorded around each labelled launch
     bool prof = false;
+    int gather_shape = 0;  // -1: direct global loads; >= 0: LDS tile shape
     struct Rec {
         const char* label;
         hipEvent_t a, b;
@@ -258,6 +264,8 @@ int slc_plan_create(const slc_grid_desc* d, int device, slc_plan** out) {
         return fail(SLC_EINVAL, "stretching must be 0 (symmetric) or 1 (bottom)");
     if (d->n_traj_iters < 0) return fail(SLC_EINVAL, "n_traj_iters must be >= 0");
     if (!d->z_f || !d->z_c) return fail(SLC_EINVAL, "z_f / z_c host tables required");
+    if ((double)(d->nx + 2) * (d->ny + 2) * (d->nz + 2) >= 2147483647.0)
+        return fail(SLC_EINVAL, "grid too large for one plan ((nx+2)(ny+2)(nz+2) >= 2^31)");
 
     DeviceGuard dg(device);
     std::call_once(g_fft_once, [] { rocfft_setup(); });
@@ -343,8 +351,39 @@ int slc_plan_create(const slc_grid_desc* d, int device, slc_plan** out) {
     p->dhat = (double*)ptr;
     if ((rc = dev_alloc(p, sizeof(double) * nmodes * nz, &ptr))) { slc_plan_destroy(p); return rc; }
     p->cpw = (double*)ptr;
-    if ((rc = dev_alloc(p, sizeof(double) * 3 * (nz + 2), &ptr))) { slc_plan_destroy(p); return rc; }
+    if ((rc = dev_alloc(p, sizeof(double) * 9 * (nz + 2), &ptr))) { slc_plan_destroy(p); return rc; }
     p->ztab = (double*)ptr;
+    {
+        const size_t fsz[3] = {(size_t)(nx + 1) * (ny + 2) * (nz + 2),
+                               (size_t)(nx + 2) * (ny + 1) * (nz + 2),
+                               (size_t)(nx + 2) * (ny + 2) * (nz + 1)};
+        for (int c = 0; c < 3; ++c) {
+            if ((rc = dev_alloc(p, sizeof(double) * fsz[c], &ptr))) { slc_plan_destroy(p); return rc; }
+            p->us[c] = (double*)ptr;
+        }
+        if ((rc = dev_alloc(p, sizeof(double) * 3 * (size_t)nx * ny * nz, &ptr))) {
+            slc_plan_destroy(p);
+            return rc;
+        }
+        p->nfoot = (double*)ptr;
+        for (int c = 0; c < 3; ++c) {
+            const int nzi = c == 2 ? nz - 1 : nz;
+            void *a = nullptr, *b = nullptr, *q = nullptr;
+            if ((rc = dev_alloc(p, sizeof(double) * nx, &a)) ||
+                (rc = dev_alloc(p, sizeof(double) * ny, &b)) ||
+                (rc = dev_alloc(p, sizeof(int) * (nzi > 0 ? nzi : 1), &q))) {
+                slc_plan_destroy(p);
+                return rc;
+            }
+            p->at[c] = ArrivalTabs{(const double*)a, (const double*)b, (const int*)q};
+            hipError_t e = run_arrival_tabs(g, c, (double*)a, (double*)b, (int*)q, nullptr);
+            if (e == hipSuccess) e = hipDeviceSynchronize();
+            if (e != hipSuccess) {
+                slc_plan_destroy(p);
+                return fail(SLC_EHIP, "arrival tables: %s", hipGetErrorString(e));
+            }
+        }
+    }
     p->max_partials = bulk_partials_needed(g);
     if ((rc = dev_alloc(p, sizeof(double) * p->max_partials, &ptr))) { slc_plan_destroy(p); return rc; }
     p->partials = (double*)ptr;
@@ -419,8 +458,18 @@ int slc_gather(slc_plan* p, int comp, const double* field, int layout, const dou
         fsx = (long)p->ny * NZ;
     }
     Timed t(p, kLabel[comp][4], stream);
-    HIP_OK(run_gather(p->g, comp, field, fsx, fsy, 0, 0, xd, yd, zd, out,
-                      reinterpret_cast<unsigned long long*>(nclamp), S(stream)));
+    if (layout == 0 && p->gather_shape == 0) {
+        TrajVel V{{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+        HIP_OK(run_tile_step(p->g, comp, 4, p->at[comp], V, field, nullptr, 0.0, out,
+                             const_cast<double*>(xd), const_cast<double*>(yd),
+                             const_cast<double*>(zd),
+                             reinterpret_cast<unsigned long long*>(nclamp), S(stream)));
+    } else if (layout == 0 && p->gather_shape > 0)
+        HIP_OK(run_gather_tiled(p->g, comp, p->gather_shape, field, xd, yd, zd, out,
+                                reinterpret_cast<unsigned long long*>(nclamp), S(stream)));
+    else
+        HIP_OK(run_gather(p->g, comp, field, fsx, fsy, 0, 0, xd, yd, zd, out,
+                          reinterpret_cast<unsigned long long*>(nclamp), S(stream)));
     return SLC_OK;
 }
 
@@ -432,6 +481,16 @@ int slc_stencil(slc_plan* p, int comp, const double* xd, const double* yd, const
     DeviceGuard dg(p->device);
     HIP_OK(run_stencil(p->g, comp, xd, yd, zd, ix, iy, k0,
                        reinterpret_cast<unsigned long long*>(nclamp), S(stream)));
+    return SLC_OK;
+}
+
+int slc_locate_z(slc_plan* p, int comp, const double* z, int64_t n, int32_t* m0_exact,
+                 int32_t* m0_fast, void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (comp < 0 || comp > 2) return fail(SLC_EINVAL, "comp must be 0, 1 or 2");
+    if (!z || !m0_exact || !m0_fast || n < 0) return fail(SLC_EINVAL, "bad array");
+    DeviceGuard dg(p->device);
+    HIP_OK(run_locate(p->g, comp, z, (long)n, m0_exact, m0_fast, S(stream)));
     return SLC_OK;
 }
 
@@ -448,7 +507,11 @@ int slc_advect(slc_plan* p, const double* u, const double* v, const double* w, c
     auto* ncl = reinterpret_cast<unsigned long long*>(nclamp);
     for (int c = 0; c < 3; ++c) {
         Timed t(p, kLabel[c][3], stream);
-        HIP_OK(run_advect(p->g, c, V, F[c], dt, O[c], ncl, S(stream)));
+        if (p->gather_shape >= 0)
+            HIP_OK(run_tile_step(p->g, c, 0, p->at[c], V, F[c], nullptr, dt, O[c], nullptr,
+                                 nullptr, nullptr, ncl, S(stream)));
+        else
+            HIP_OK(run_advect(p->g, c, V, F[c], dt, O[c], ncl, S(stream)));
     }
     return SLC_OK;
 }
@@ -465,7 +528,11 @@ int slc_bdf1_predict(slc_plan* p, const double* u, const double* v, const double
     auto* ncl = reinterpret_cast<unsigned long long*>(nclamp);
     for (int c = 0; c < 3; ++c) {
         Timed t(p, kLabel[c][2], stream);
-        HIP_OK(run_bdf1(p->g, c, V, F[c], dt, O[c], ncl, S(stream)));
+        if (p->gather_shape >= 0)
+            HIP_OK(run_tile_step(p->g, c, 1, p->at[c], V, F[c], nullptr, dt, O[c], nullptr,
+                                 nullptr, nullptr, ncl, S(stream)));
+        else
+            HIP_OK(run_bdf1(p->g, c, V, F[c], dt, O[c], ncl, S(stream)));
     }
     return SLC_OK;
 }
@@ -478,11 +545,27 @@ int slc_bdf2_predict(slc_plan* p, const double* u, const double* v, const double
     if (!u || !v || !w || !unm1 || !vnm1 || !wnm1 || !ustar || !vstar || !wstar)
         return fail(SLC_EINVAL, "null array");
     DeviceGuard dg(p->device);
-    TrajVel V{{u, v, w}, {unm1, vnm1, wnm1}};
     const double* Fn[3] = {u, v, w};
     const double* Fm[3] = {unm1, vnm1, wnm1};
     double* O[3] = {ustar, vstar, wstar};
     auto* ncl = reinterpret_cast<unsigned long long*>(nclamp);
+    if (p->gather_shape >= 0) {
+        {
+            Timed t(p, "ustar", stream);
+            HIP_OK(run_ustar(p->g, p->us, Fn, Fm, S(stream)));
+        }
+        TrajVel V{{p->us[0], p->us[1], p->us[2]}, {nullptr, nullptr, nullptr}};
+        const long N = (long)p->nx * p->ny * p->nz;
+        for (int c = 0; c < 3; ++c)
+            for (int pass = 0; pass < 2; ++pass) {
+                Timed t(p, kLabel[c][pass], stream);
+                HIP_OK(run_tile_step(p->g, c, 2 + pass, p->at[c], V, Fn[c], Fm[c], dt, O[c],
+                                     p->nfoot, p->nfoot + N, p->nfoot + 2 * N, ncl,
+                                     S(stream)));
+            }
+        return SLC_OK;
+    }
+    TrajVel V{{u, v, w}, {unm1, vnm1, wnm1}};
     for (int pass = 0; pass < 2; ++pass)
         for (int c = 0; c < 3; ++c) {
             Timed t(p, kLabel[c][pass], stream);
@@ -498,8 +581,14 @@ int slc_zsolve(slc_plan* p, int comp, double* field, double dt, double theta, vo
     if (!(theta > 0.0 && theta <= 1.0)) return fail(SLC_EINVAL, "theta must be in (0, 1]");
     if (comp == 2 && p->nz < 3) return fail(SLC_EINVAL, "w z-solve needs nz >= 3");
     DeviceGuard dg(p->device);
+    // the factorisation depends only on (dt, theta): rebuilt when they change
+    bool refactor = !(p->zvalid[comp] && p->zkey[comp][0] == dt && p->zkey[comp][1] == theta);
+    p->zvalid[comp] = true;
+    p->zkey[comp][0] = dt;
+    p->zkey[comp][1] = theta;
     Timed t(p, kZLabel[comp], stream);
-    HIP_OK(run_zsolve(p->g, comp, field, dt, theta, p->d_dzc, p->d_dzf, p->ztab, S(stream)));
+    HIP_OK(run_zsolve(p->g, comp, field, dt, theta, p->d_dzc, p->d_dzf,
+                      p->ztab + (size_t)comp * 3 * (p->nz + 2), refactor, S(stream)));
     return SLC_OK;
 }
 
@@ -575,6 +664,18 @@ int slc_bulk_force(slc_plan* p, double* u, double* v, double* w, double U_bulk, 
     HIP_OK(run_bulk(p->g, u, v, w, p->d_cv, p->total_volume, U_bulk, dt, 1, p->partials,
                     p->max_partials, out, S(stream)));
     return SLC_OK;
+}
+
+int slc_set_option(slc_plan* p, int option, int value) {
+    if (int rc = check_plan(p)) return rc;
+    switch (option) {
+        case SLC_OPT_GATHER_TILE:
+            if (value < -1 || value > 3) return fail(SLC_EINVAL, "gather tile shape %d", value);
+            p->gather_shape = value;
+            return SLC_OK;
+        default:
+            return fail(SLC_EINVAL, "unknown option %d", option);
+    }
 }
 
 int slc_profile_enable(slc_plan* p, int on) {

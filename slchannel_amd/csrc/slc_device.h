@@ -89,6 +89,24 @@ __device__ inline int pymod(long a, int n) {
     return (int)(r < 0 ? r + n : r);
 }
 
+// Python-style i mod n for 32-bit i.  Feet are CFL-bounded, so i is almost
+// always within one period of [0, n): two compares instead of a division.
+__device__ inline int wrap_idx(int i, int n) {
+    if ((unsigned)i < (unsigned)n) return i;
+    if (i < 0 && i >= -n) return i + n;
+    if (i >= n && i < 2 * n) return i - n;
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+// floor(s) as an int; |s| is clamped to 2^30 first so the conversion is
+// defined for any input (identical to (long)floor(s) for real feet).
+__device__ inline int floor_int(double f) {
+    const double lim = 1073741824.0;
+    f = f < -lim ? -lim : (f > lim ? lim : f);
+    return (int)f;
+}
+
 // Continuous face index kf (semilag.py:400-411).
 __device__ inline double face_coord(const Grid& g, double z) {
     if (g.symmetric) {
@@ -107,11 +125,44 @@ __device__ inline double face_coord(const Grid& g, double z) {
 // Face-interval index c = clamp(floor(kf), 0, nz-1) (semilag.py:415-416).
 __device__ inline int face_cell(const Grid& g, double z) {
     double kf = face_coord(g, z);
-    double f = floor(kf);
-    long c = (long)f;
+    int c = floor_int(floor(kf));
     if (c < 0) c = 0;
     if (c > g.nz - 1) c = g.nz - 1;
-    return (int)c;
+    return c;
+}
+
+// face_cell without the double-precision atanh.  A float32 estimate of the
+// inverse map is refined against the face table to the interval
+// z_f[c] <= z < z_f[c+1] (clamped to [0, nz-1]).  That interval is
+// floor(kf(z)) except where kf(z) rounds across an integer, which can only
+// happen for z within a few 1e-13 cell heights of a face (the analytic
+// inverse of a face table entry is k to ~1e-13); within 1e-10 of a cell height
+// of a face the analytic path itself runs, so the result always equals
+// face_cell(g, z).  z must lie in [z_lo, z_hi].
+__device__ inline int face_cell_fast(const Grid& g, double z) {
+    const int nz = g.nz;
+    float kfe;
+    if (g.symmetric) {
+        float a = (float)((2.0 * z / g.Lz - 1.0) * g.tanh_g);
+        a = fminf(fmaxf(a, -0.9999999f), 0.9999999f);
+        float at = 0.5f * __logf((1.0f + a) / (1.0f - a));
+        kfe = (at / (float)g.gamma + 1.0f) * (0.5f * (float)nz);
+    } else {
+        float a = (float)((1.0 - z / g.Lz) * g.tanh_g);
+        a = fminf(fmaxf(a, -0.9999999f), 0.9999999f);
+        float at = 0.5f * __logf((1.0f + a) / (1.0f - a));
+        kfe = (1.0f - at / (float)g.gamma) * (float)nz;
+    }
+    int c = (int)floorf(fminf(fmaxf(kfe, 0.0f), (float)(nz - 1)));
+    const double* zf = g.z_f;
+    int guard = 0;
+    while (c > 0 && z < zf[c] && guard < 8) { --c; ++guard; }
+    while (c < nz - 1 && z >= zf[c + 1] && guard < 16) { ++c; ++guard; }
+    const double lo = zf[c], hi = zf[c + 1];
+    const double band = 1e-10 * (hi - lo);
+    const bool bracketed = (c == 0 || z >= lo) && (c == nz - 1 || z < hi);
+    if (!bracketed || z - lo <= band || hi - z <= band) return face_cell(g, z);
+    return c;
 }
 
 // Interval index m0 of the component's z nodes containing z (semilag.py:413-420).
@@ -147,6 +198,25 @@ __device__ inline void uniform_weights(double t, double* w) {
     }
 }
 
+// Single uniform weight w[m] of uniform_weights<P> for a runtime m (same
+// operation order, so the value is identical).
+template <int P>
+__device__ inline double uniform_weight_m(double t, int m) {
+    constexpr int h = P / 2 - 1;
+    double den = 1.0;
+    for (int l = 0; l < P; ++l)
+        if (l != m) den *= (double)((m - h) - (l - h));
+    double num = 0.0;
+    bool first = true;
+    for (int l = 0; l < P; ++l) {
+        if (l == m) continue;
+        double d = t - (double)(l - h);
+        num = first ? d : num * d;
+        first = false;
+    }
+    return num * (1.0 / den);
+}
+
 // Nonuniform z weights against the actual nodes (semilag.py:441-454).
 template <int P>
 __device__ inline void z_weights(const double* zn, const double* dinv, double z, int k0,
@@ -169,9 +239,11 @@ __device__ inline void z_weights(const double* zn, const double* dinv, double z,
 }
 
 // Linear address of node (ib, jb, k) of component C inside its ghost-shaped array.
+// (32-bit: plans keep every field below 2^31 elements)
 template <int C>
-__device__ inline long node_addr(const Grid& g, int ib, int jb, int k) {
-    return ((long)(ib + Comp<C>::ox) * Comp<C>::SY(g) + (jb + Comp<C>::oy)) * Comp<C>::SZ(g) + k;
+__device__ inline int node_addr(const Grid& g, int ib, int jb, int k) {
+    return ((ib + Comp<C>::ox) * (int)Comp<C>::SY(g) + (jb + Comp<C>::oy)) *
+               (int)Comp<C>::SZ(g) + k;
 }
 
 // Arrival point of flat interior index n (z fastest) of component C
@@ -179,11 +251,12 @@ __device__ inline long node_addr(const Grid& g, int ib, int jb, int k) {
 template <int C>
 __device__ inline void arrival(const Grid& g, long n, double& xa, double& ya, double& za, int& I,
                                int& J, int& K) {
-    const int nzi = Comp<C>::NZI(g);
-    int k = (int)(n % nzi);
-    long r = n / nzi;
-    int j = (int)(r % g.ny);
-    int i = (int)(r / g.ny);
+    const unsigned nzi = (unsigned)Comp<C>::NZI(g);
+    const unsigned nn = (unsigned)n;  // plans keep nx*ny*nz < 2^31
+    int k = (int)(nn % nzi);
+    unsigned r = nn / nzi;
+    int j = (int)(r % (unsigned)g.ny);
+    int i = (int)(r / (unsigned)g.ny);
     double ar = (double)(i + 1), aj = (double)(j + 1);
     if (C == 0) {
         xa = ar * g.dx;
@@ -225,8 +298,9 @@ __device__ inline double trilinear(const Grid& g, const TrajVel& V, double xdx, 
     double sy = ydy + Comp<C>::shift_y;
     double iy0 = floor(sy);
     double ty = sy - iy0;
-    int ixa = pymod((long)ix0, g.nx), ixb = pymod((long)ix0 + 1, g.nx);
-    int iya = pymod((long)iy0, g.ny), iyb = pymod((long)iy0 + 1, g.ny);
+    const int fx = floor_int(ix0), fy = floor_int(iy0);
+    int ixa = wrap_idx(fx, g.nx), ixb = wrap_idx(fx + 1, g.nx);
+    int iya = wrap_idx(fy, g.ny), iyb = wrap_idx(fy + 1, g.ny);
     double wx[2], wy[2], wz[2];
     // order-2 uniform weights: (t - 1) * (-1), t * 1
     wx[0] = (tx - 1.0) * (-1.0);
@@ -250,7 +324,7 @@ __device__ inline double trilinear(const Grid& g, const TrajVel& V, double xdx, 
     for (int i = 0; i < 2; ++i) {
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            long base = node_addr<C>(g, ix[i], iy[j], k0);
+            const int base = node_addr<C>(g, ix[i], iy[j], k0);
             double wij = wx[i] * wy[j];
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
@@ -275,7 +349,7 @@ __device__ inline void foot(const Grid& g, const TrajVel& V, double xa, double y
         double zc = clampd(zm, g.z_lo, g.z_hi);
         double xdx = xm / g.dx;
         double ydy = ym / g.dy;
-        int cell = face_cell(g, zc);
+        int cell = face_cell_fast(g, zc);
         double us = trilinear<0>(g, V, xdx, ydy, zc, cell);
         double vs = trilinear<1>(g, V, xdx, ydy, zc, cell);
         double ws = trilinear<2>(g, V, xdx, ydy, zc, cell);
@@ -306,8 +380,7 @@ __device__ inline void build_stencil(const Grid& g, double x, double y, double z
     double sy = y / g.dy + Comp<C>::shift_y;
     double iy0 = floor(sy);
     double ty = sy - iy0;
-    long bx = (long)ix0 - h, by = (long)iy0 - h;
-    int x0 = pymod(bx, g.nx), y0 = pymod(by, g.ny);
+    int x0 = wrap_idx(floor_int(ix0) - h, g.nx), y0 = wrap_idx(floor_int(iy0) - h, g.ny);
 #pragma unroll
     for (int l = 0; l < P; ++l) {
         int xi = x0 + l;
@@ -319,7 +392,7 @@ __device__ inline void build_stencil(const Grid& g, double x, double y, double z
     }
     uniform_weights<P>(tx, s.wx);
     uniform_weights<P>(ty, s.wy);
-    int m0 = locate_from_cell<C>(g, face_cell(g, z), z);
+    int m0 = locate_from_cell<C>(g, face_cell_fast(g, z), z);
     int k0 = m0 - h;
     const int NZ = Comp<C>::NZ(g);
     if (k0 < 0) k0 = 0;
@@ -347,6 +420,29 @@ __device__ inline double gather(const Grid& g, const double* __restrict__ F,
         acc = fma(s.wx[i], ai, acc);
     }
     return acc;
+}
+
+// Unwrapped integer stencil base of a (clamped) foot: x/y node index of stencil
+// offset 0 before the periodic wrap, and the clamped z base k0.  Same
+// arithmetic as build_stencil.
+template <int C, int P>
+__device__ inline void stencil_base(const Grid& g, double x, double y, double z, int& bx,
+                                    int& by, int& k0, double& tx, double& ty) {
+    constexpr int h = P / 2 - 1;
+    double sx = x / g.dx + Comp<C>::shift_x;
+    double ix0 = floor(sx);
+    tx = sx - ix0;
+    double sy = y / g.dy + Comp<C>::shift_y;
+    double iy0 = floor(sy);
+    ty = sy - iy0;
+    bx = floor_int(ix0) - h;
+    by = floor_int(iy0) - h;
+    int m0 = locate_from_cell<C>(g, face_cell_fast(g, z), z);
+    int kb = m0 - h;
+    const int NZ = Comp<C>::NZ(g);
+    if (kb < 0) kb = 0;
+    if (kb > NZ - P) kb = NZ - P;
+    k0 = kb;
 }
 
 // Explicit xy-diffusion nu*(d2x + d2y) at ghost index (I, J, K)

@@ -23,6 +23,11 @@ hipError_t run_departure(const Grid& g, int comp, const TrajVel& V, double dt, d
 hipError_t run_gather(const Grid& g, int comp, const double* F, long fsx, long fsy, int fox,
                       int foy, const double* xd, const double* yd, const double* zd, double* out,
                       unsigned long long* ncl, hipStream_t s);
+hipError_t run_gather_tiled(const Grid& g, int comp, int shape, const double* F,
+                            const double* xd, const double* yd, const double* zd, double* out,
+                            unsigned long long* ncl, hipStream_t s);
+hipError_t run_locate(const Grid& g, int comp, const double* z, long n, int* exact, int* fast,
+                      hipStream_t s);
 hipError_t run_stencil(const Grid& g, int comp, const double* xd, const double* yd,
                        const double* zd, int* ix, int* iy, int* k0, unsigned long long* ncl,
                        hipStream_t s);
@@ -34,13 +39,32 @@ hipError_t run_bdf2(const Grid& g, int comp, int pass, const TrajVel& V, const d
                     const double* Fm, double dt, double* out, unsigned long long* ncl,
                     hipStream_t s);
 
+// Per-component arrival tables: x/dx (nx), y/dy (ny), face cell of z (nzi).
+struct ArrivalTabs {
+    const double* xdx;
+    const double* ydy;
+    const int* cell;
+};
+
+// LDS-tiled step kernels (slc_step.hip); mode: 0 advect, 1 bdf1, 2 far,
+// 3 near, 4 gather (feet given)
+hipError_t run_tile_step(const Grid& g, int comp, int mode, const ArrivalTabs& A,
+                         const TrajVel& V, const double* F, const double* Fm, double dt,
+                         double* out, double* fx, double* fy, double* fz,
+                         unsigned long long* ncl, hipStream_t s);
+hipError_t run_arrival_tabs(const Grid& g, int comp, double* xdx, double* ydy, int* cell,
+                            hipStream_t s);
+hipError_t run_ustar(const Grid& g, double* const o[3], const double* const a[3],
+                     const double* const b[3], hipStream_t s);
+
 // field operators (slc_ops.hip)
 hipError_t run_fill_u64(unsigned long long* p, long n, unsigned long long v, hipStream_t s);
 hipError_t run_fill_f64(double* p, long n, double v, hipStream_t s);
 hipError_t run_bc(const Grid& g, double* u, double* v, double* w, hipStream_t s);
 hipError_t run_bc_p(const Grid& g, double* p, hipStream_t s);
 hipError_t run_zsolve(const Grid& g, int comp, double* f, double dt, double theta,
-                      const double* dz_c, const double* dz_f, double* tab, hipStream_t s);
+                      const double* dz_c, const double* dz_f, double* tab, bool refactor,
+                      hipStream_t s);
 hipError_t run_div_planes(const Grid& g, const double* u, const double* v, const double* w,
                           const double* dz_f, double scale, double* planes, hipStream_t s);
 hipError_t run_div_field(const Grid& g, const double* u, const double* v, const double* w,

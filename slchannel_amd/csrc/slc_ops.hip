@@ -230,14 +230,24 @@ __global__ __launch_bounds__(256) void zsolve_kernel(Grid g, double* __restrict_
     const long col0 = (long)blockIdx.x * kZCols;
     const int tid = threadIdx.x;
 
+    // column base addresses (one 32-bit division per column, not per element)
+    __shared__ long cbase[kZColsMax];
+    if (tid < kZCols) {
+        const long gc = col0 + tid;
+        long a = -1;
+        if (gc < ncols) {
+            const unsigned ug = (unsigned)gc;
+            const int I = 1 + (int)(ug / (unsigned)g.ny), J = 1 + (int)(ug % (unsigned)g.ny);
+            a = ((long)I * SY + J) * SZ;
+        }
+        cbase[tid] = a;
+    }
+    __syncthreads();
     // coalesced load of kZCols whole columns (ghost ends included)
     for (int e = tid; e < kZCols * SZ; e += blockDim.x) {
-        int c = e / SZ, k = e % SZ;
-        long gc = col0 + c;
-        if (gc < ncols) {
-            int I = 1 + (int)(gc / g.ny), J = 1 + (int)(gc % g.ny);
-            lds[c * ld + k] = F[((long)I * SY + J) * SZ + k];
-        }
+        const int c = e / SZ, k = e - (e / SZ) * SZ;
+        const long a = cbase[c];
+        if (a >= 0) lds[c * ld + k] = F[a + k];
     }
     __syncthreads();
 
@@ -298,19 +308,18 @@ __global__ __launch_bounds__(256) void zsolve_kernel(Grid g, double* __restrict_
     __syncthreads();
 
     for (int e = tid; e < kZCols * SZ; e += blockDim.x) {
-        int c = e / SZ, k = e % SZ;
-        long gc = col0 + c;
-        if (gc < ncols && k >= 1 && k <= n) {
-            int I = 1 + (int)(gc / g.ny), J = 1 + (int)(gc % g.ny);
-            F[((long)I * SY + J) * SZ + k] = lds[c * ld + k];
-        }
+        const int c = e / SZ, k = e - (e / SZ) * SZ;
+        const long a = cbase[c];
+        if (a >= 0 && k >= 1 && k <= n) F[a + k] = lds[c * ld + k];
     }
 }
 
 hipError_t run_zsolve(const Grid& g, int comp, double* f, double dt, double theta,
-                      const double* dz_c, const double* dz_f, double* tab, hipStream_t s) {
-    hipLaunchKernelGGL(zfactor_kernel, dim3(1), dim3(64), 0, s, g, comp, dt, theta, dz_c, dz_f,
-                       tab);
+                      const double* dz_c, const double* dz_f, double* tab, bool refactor,
+                      hipStream_t s) {
+    if (refactor)
+        hipLaunchKernelGGL(zfactor_kernel, dim3(1), dim3(64), 0, s, g, comp, dt, theta, dz_c,
+                           dz_f, tab);
     double beta = theta < 1.0 ? (1.0 - theta) * dt * g.nu : 0.0;
     long ncols = (long)g.nx * g.ny;
     int SZ = comp == 2 ? g.nz + 1 : g.nz + 2;
@@ -395,9 +404,10 @@ __global__ void div_field_kernel(Grid g, const double* __restrict__ u,
     long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long N = (long)g.nx * g.ny * g.nz;
     if (n >= N) return;
-    int k = (int)(n % g.nz);
-    long r = n / g.nz;
-    int j = (int)(r % g.ny), i = (int)(r / g.ny);
+    const unsigned un = (unsigned)n;
+    int k = (int)(un % (unsigned)g.nz);
+    unsigned r = un / (unsigned)g.nz;
+    int j = (int)(r % (unsigned)g.ny), i = (int)(r / (unsigned)g.ny);
     div[n] = divergence_at(g, u, v, w, dz_f, i + 1, j + 1, k + 1);
 }
 
@@ -576,9 +586,10 @@ __global__ void project_p_kernel(Grid g, double* __restrict__ u, double* __restr
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= (long)nx * ny * nz) return;
-    const int k = (int)(n % nz);
-    const long r = n / nz;
-    const int j = (int)(r % ny), i = (int)(r / ny);
+    const unsigned un = (unsigned)n;
+    const int k = (int)(un % (unsigned)nz);
+    const unsigned r = un / (unsigned)nz;
+    const int j = (int)(r % (unsigned)ny), i = (int)(r / (unsigned)ny);
     const int I = i + 1, J = j + 1, K = k + 1;
     const long PSY = ny + 2, PSZ = nz + 2;
     const long pc = ((long)I * PSY + J) * PSZ + K;
@@ -641,9 +652,10 @@ __global__ __launch_bounds__(kRedBlock) void bulk_partial_kernel(Grid g, double*
     const double corr = shift ? scal[0] : 0.0;
     double acc = 0.0;
     for (long n = beg + threadIdx.x; n < end; n += kRedBlock) {
-        int k = (int)(n % g.nz);
-        long r = n / g.nz;
-        int j = (int)(r % g.ny), i = (int)(r / g.ny);
+        const unsigned un = (unsigned)n;
+        int k = (int)(un % (unsigned)g.nz);
+        unsigned r = un / (unsigned)g.nz;
+        int j = (int)(r % (unsigned)g.ny), i = (int)(r / (unsigned)g.ny);
         long ui = ((long)(i + 1) * (g.ny + 2) + (j + 1)) * (g.nz + 2) + (k + 1);
         double val = u[ui];
         if (shift) {

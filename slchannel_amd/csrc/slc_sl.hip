@@ -121,6 +121,27 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(Grid g, const double* _
     if (nclamp) block_count(nclamp, clamped);
 }
 
+// z-locate diagnostic: node-interval index m0 (semilag.py:413-420) by the
+// analytic path and by the fast path used in the kernels.
+template <int C>
+__global__ void locate_kernel(Grid g, const double* __restrict__ z, long n, int* exact,
+                              int* fast) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double zc = clampd(z[i], g.z_lo, g.z_hi);
+    exact[i] = locate_from_cell<C>(g, face_cell(g, zc), zc);
+    fast[i] = locate_from_cell<C>(g, face_cell_fast(g, zc), zc);
+}
+
+hipError_t run_locate(const Grid& g, int comp, const double* z, long n, int* exact, int* fast,
+                      hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    dim3 grid((unsigned)((n + 255) / 256));
+    if (comp == 2) hipLaunchKernelGGL(locate_kernel<2>, grid, dim3(256), 0, s, g, z, n, exact, fast);
+    else hipLaunchKernelGGL(locate_kernel<0>, grid, dim3(256), 0, s, g, z, n, exact, fast);
+    return hipGetLastError();
+}
+
 template <int C>
 __device__ inline long out_addr(const Grid& g, int I, int J, int K) {
     return ((long)I * Comp<C>::SY(g) + J) * Comp<C>::SZ(g) + K;

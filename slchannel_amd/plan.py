@@ -40,10 +40,12 @@ class HipPlan:
         if not torch.cuda.is_available():
             raise RuntimeError("the HIP layer needs a ROCm GPU (torch.cuda.is_available() is False)")
         lib = _lib.load()
-        self.device = torch.device("cuda", torch.cuda.current_device()) if device is None \
-            else torch.device(device)
-        if self.device.type != "cuda":
-            raise ValueError(f"HipPlan needs a cuda (HIP) device, got {self.device}")
+        dev = torch.device("cuda") if device is None else torch.device(device)
+        if dev.type != "cuda":
+            raise ValueError(f"HipPlan needs a cuda (HIP) device, got {dev}")
+        if dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
+        self.device = dev
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.Lx, self.Ly, self.Lz = float(Lx), float(Ly), float(Lz)
         self.order = int(order)
@@ -93,6 +95,12 @@ class HipPlan:
         _lib.check(self._lib.slc_plan_workspace_bytes(self._h, ctypes.byref(out)),
                    "slc_plan_workspace_bytes")
         return out.value
+
+    OPT_GATHER_TILE = 1
+
+    def set_option(self, option, value):
+        """Tuning knob (include/slc.h: SLC_OPT_*)."""
+        _lib.check(self._lib.slc_set_option(self._h, int(option), int(value)), "slc_set_option")
 
     def profile(self, on=True):
         """Bracket every labelled launch with hipEvents (see profile_report)."""
@@ -184,6 +192,15 @@ class HipPlan:
         _lib.check(self._lib.slc_stencil(self._h, COMPS.index(comp), _ptr(xd), _ptr(yd),
                                          _ptr(zd), _ptr(ix), _ptr(iy), _ptr(k0),
                                          self._counter(nclamp), self._stream()), "slc_stencil")
+
+    def locate_z(self, comp, z, exact, fast):
+        """Node-interval index m0 of z (semilag.py:413-420): analytic and fast paths."""
+        n = z.numel()
+        self._chk(z, (n,), "z")
+        self._chk(exact, (n,), "exact", torch.int32)
+        self._chk(fast, (n,), "fast", torch.int32)
+        _lib.check(self._lib.slc_locate_z(self._h, COMPS.index(comp), _ptr(z), n, _ptr(exact),
+                                          _ptr(fast), self._stream()), "slc_locate_z")
 
     def advect(self, u, v, w, um, vm, wm, dt, ustar, vstar, wstar, nclamp=None):
         self._uvw(u, v, w)

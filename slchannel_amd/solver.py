@@ -77,6 +77,8 @@ class SLChannelFlow:
         if device is None or torch.device(device).type != "cuda":
             raise RuntimeError("SLChannelFlow (HIP layer) needs a ROCm GPU device")
         self.device = torch.device(device)
+        if self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
         if not env.USE_HIP:
             raise RuntimeError("SLCHANNEL_HIP=0: the HIP layer is disabled")
 

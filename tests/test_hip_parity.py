@@ -318,3 +318,99 @@ def test_step_vs_oracle_seeded(check, order):
               f"{int(flow.sl.n_clamped_last.item())} vs {co.n_clamped_last}")
     check("fields vs oracle over 8 steps", worst < 1e-12,
           f"max rel={worst:.1e} per step={per_step} dt={dt:.4f}")
+
+
+@pytest.mark.parametrize("order", [4, 6])
+def test_tiled_gather_matches_direct(check, order):
+    """The LDS-tiled gather must reproduce the direct gather bit for bit
+    (same stencils, same accumulation order), for every tile shape."""
+    c = _random_case(40, 36, 48, order, "symmetric", seed=31, cfl=3.0)
+    plan = make_plan(c)
+    mids = [cuda(a) for a in c["mids"]]
+    for ic, comp in enumerate("uvw"):
+        n = plan.n_arrival(comp)
+        xd, yd, zd = (torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(3))
+        plan.departure(comp, *mids, 2.0 * c["dt"], xd, yd, zd)
+        f = cuda(c["fields"][ic])
+        ref = torch.empty(n, dtype=torch.float64, device="cuda")
+        ncl0 = torch.zeros(1, dtype=torch.int64, device="cuda")
+        plan.set_option(plan.OPT_GATHER_TILE, -1)
+        plan.gather(comp, f, xd, yd, zd, ref, ncl0)
+        for shape in range(4):
+            plan.set_option(plan.OPT_GATHER_TILE, shape)
+            out = torch.empty(n, dtype=torch.float64, device="cuda")
+            ncl = torch.zeros(1, dtype=torch.int64, device="cuda")
+            plan.gather(comp, f, xd, yd, zd, out, ncl)
+            check(f"{comp} shape {shape} bit-identical", torch.equal(out, ref) and
+                  int(ncl.item()) == int(ncl0.item()),
+                  f"max|d|={(out - ref).abs().max().item():.1e} ncl={int(ncl.item())}")
+
+
+@pytest.mark.parametrize("order", [4, 6])
+def test_tiled_step_kernels_match_direct(check, order):
+    """advect, BDF1 and BDF2 predictors: LDS-tiled kernels (default) against
+    the direct global-load kernels -- identical results and clamp counts."""
+    c = _random_case(40, 36, 48, order, "symmetric", seed=17, cfl=3.0)
+    plan = make_plan(c, nu=1e-3)
+    f = [cuda(a) for a in c["fields"]]
+    m = [cuda(a) for a in c["mids"]]
+    dt = c["dt"]
+    res = {}
+    for shape in (-1, 0):
+        plan.set_option(plan.OPT_GATHER_TILE, shape)
+        outs = {}
+        for name in ("advect", "bdf1", "bdf2"):
+            o = [torch.zeros(plan.shapes[k], dtype=torch.float64, device="cuda") for k in "uvw"]
+            ncl = torch.zeros(1, dtype=torch.int64, device="cuda")
+            if name == "advect":
+                plan.advect(*f, *m, dt, *o, ncl)
+            elif name == "bdf1":
+                plan.bdf1_predict(*f, dt, *o, ncl)
+            else:
+                plan.bdf2_predict(*f, *m, dt, *o, ncl)
+            outs[name] = (o, int(ncl.item()))
+        res[shape] = outs
+    plan.set_option(plan.OPT_GATHER_TILE, 0)
+    for name in ("advect", "bdf1", "bdf2"):
+        (a, na), (b, nb) = res[0][name], res[-1][name]
+        same = all(torch.equal(x, y) for x, y in zip(a, b))
+        dev = max((x - y).abs().max().item() for x, y in zip(a, b))
+        check(f"{name}: tiled == direct", same and na == nb, f"max|d|={dev:.1e} ncl={na},{nb}")
+
+
+@pytest.mark.parametrize("stretching,gamma,nz", [("symmetric", 1.5, 256), ("symmetric", 2.6, 180),
+                                                 ("bottom", 1.8, 512)])
+def test_fast_z_locate_equals_analytic(check, stretching, gamma, nz):
+    """The kernels' table-refined z-locate must equal the analytic inverse-map
+    path everywhere, including points within ulps of every face; the analytic
+    path must equal the oracle's off exact ties."""
+    from slchannel_amd import HipPlan
+    from slchannel_amd.grid import generate_grid
+
+    z_f, z_c, _, _ = (t.numpy() for t in generate_grid(gamma, nz, 2.0, stretching_type=stretching))
+    plan = HipPlan(8, 8, nz, 4.0, 4.0, 2.0, z_f, z_c, gamma, stretching_type=stretching, order=6)
+    dz = np.diff(z_f)
+    pts = [np.random.default_rng(0).uniform(0.0, 2.0, 200000), z_c, z_f]
+    for k_ulp in (1, 2, 4, 16, 1000):
+        pts += [np.nextafter(z_f, 10.0 * np.ones_like(z_f)) if k_ulp == 1 else
+                z_f + k_ulp * np.spacing(z_f), z_f - k_ulp * np.spacing(z_f)]
+    for rel in (1e-13, 1e-12, 1e-11, 1e-10, 3e-10, 1e-9):
+        pts += [z_f[:-1] + rel * dz, z_f[1:] - rel * dz]
+    z = np.concatenate(pts)
+    tz = cuda(z)
+    for comp in ("u", "w"):
+        ex = torch.empty(z.size, dtype=torch.int32, device="cuda")
+        fa = torch.empty_like(ex)
+        plan.locate_z(comp, tz, ex, fa)
+        ex, fa = ex.cpu().numpy(), fa.cpu().numpy()
+        check(f"{comp}: fast == analytic on {z.size} points", np.array_equal(ex, fa),
+              f"mismatches={int((ex != fa).sum())}")
+        sl = orc.SLOracle(8, 8, nz, 0.5, 0.5, 4.0, 4.0, 2.0, z_f, z_c, gamma,
+                          stretching_type=stretching, order=6)
+        sl.set_tanh_g(plan.tanh_g)
+        ref = sl.locate_z(comp, z)
+        kf = sl.face_coord(np.clip(z, sl.z_lo, sl.z_hi))
+        tie = np.abs(kf - np.round(kf)) < 1e-9
+        off = ~tie
+        check(f"{comp}: analytic == oracle off ties", np.array_equal(ex[off], ref[off]),
+              f"mismatches={int((ex[off] != ref[off]).sum())} ties={int(tie.sum())}")
