@@ -10,10 +10,11 @@ full SL-BDF2 step (predictor, z-solves, FFT projection, bulk shift); the W
 warm-up steps include the BDF1 bootstrap.  Inputs are resident in HBM.
 
 Prints ONE JSON line.  Also reported:
-  roofline      the dominant kernel (the near-foot BDF2 pass of u) timed with
-                hipEvents on its launch stream over the timed region;
-                algorithmic bytes = reads of the six velocity arrays + the far
-                value + the write, 64 B per arrival point;
+  roofline      the dominant kernel -- the far-foot BDF2 pass of u (two-foot
+                tracer + order-P gather) -- timed with hipEvents on its launch
+                stream over the timed region; algorithmic bytes per arrival
+                point: U* 3x8 + V^{n-1} 8 read, near foot 3x8 + out 8 written
+                = 64 B;
   gather_GBps   the isolated order-P gather of u (slc_gather) at 40 B/point
                 (3x8 B feet + 8 B field + 8 B out), feet from the 2dt foot;
   cpu_baseline  the CPU oracle (oracle/: numpy + OpenMP C) timed on this
@@ -42,6 +43,14 @@ def parse():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--nz", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--storage", type=int, default=64, choices=(64, 32),
+                    help="bits of the gathered field copies: 64 (default) or 32 "
+                         "(FP32-storage variant; adds an FP64 twin run and reports the "
+                         "per-field difference)")
+    ap.add_argument("--config", type=int, default=2, choices=(2, 3),
+                    help="2: 256^3 channel (default, the headline workload); "
+                         "3: 1024x1024x512 large box, CFL 4, plus the isolated gather "
+                         "with feet displaced uniformly in +-5 cells")
     return ap.parse_args()
 
 
@@ -55,19 +64,27 @@ def main():
 
     from slchannel_amd import SLChannelFlow
 
+    if args.config == 3:
+        return large_box(args)
     n, nz = args.n, args.nz or args.n
     Re_b, Re_tau, dtp = 2792.8, 180.0, 0.2
     cfg = {
         "grid": {"nx": n, "ny": n, "nz": nz},
         "domain": {"Lx": 4 * math.pi, "Ly": 2 * math.pi, "Lz": 2.0},
         "flow": {"Re": Re_b, "U_bulk": 1.0, "gamma": 1.5},
-        "sl": {"interp_order": args.order},
+        "sl": {"interp_order": args.order,
+               "interp_dtype": "fp32_accum64" if args.storage == 32 else "fp64"},
     }
     flow = SLChannelFlow(config=cfg, device="cuda:0")
     flow.initialize(seed=2000, urms=0.15, kmin=1.0, kmax=32.0, spectrum="k-5/3")
     nu = flow.nu
     u_tau = Re_tau * nu  # delta = 1
     dt = dtp * nu / (u_tau * u_tau)
+    twin = None
+    if args.storage == 32:
+        cfg64 = dict(cfg, sl={"interp_order": args.order, "interp_dtype": "fp64"})
+        twin = SLChannelFlow(config=cfg64, device="cuda:0")
+        twin.set_fields(flow.u, flow.v, flow.w, flow.p)
 
     for _ in range(args.warmup):
         flow.step_sl_bdf2(dt)
@@ -86,11 +103,11 @@ def main():
     ms_step = (t1 - t0) * 1e3 / args.steps
 
     N_u = n * n * nz
-    lab = "bdf2_near_u"
+    lab = "bdf2_far_u"
     cnt, tot = prof[lab]
     avg_ms = tot / cnt
-    bytes_near = 64.0 * N_u
-    achieved = bytes_near / (avg_ms * 1e-3) / 1e9
+    bytes_far = (64.0 if args.storage == 64 else 60.0) * N_u
+    achieved = bytes_far / (avg_ms * 1e-3) / 1e9
 
     # isolated gather of u at the 2dt foot of the current state
     xd, yd, zd = (torch.empty(N_u, dtype=torch.float64, device="cuda") for _ in range(3))
@@ -111,14 +128,24 @@ def main():
     # correctness guards on the measured run (no host sync inside the loop)
     div = flow.divergence().abs().max().item()
     ubv = ub.item()
+    fp32_err = None
+    if twin is not None:
+        for _ in range(args.warmup + args.steps):
+            twin.step_sl_bdf2(dt)
+        fp32_err = {k: (a - b).abs().max().item() / b.abs().max().item()
+                    for k, a, b in (("u", flow.u, twin.u), ("v", flow.v, twin.v),
+                                    ("w", flow.w, twin.w), ("p", flow.p, twin.p))}
+        fp32_err["steps"] = args.warmup + args.steps
 
+    traffic = measured_traffic(n, nz, args.order) if args.storage == 64 else None
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(flow, dt)
 
     kernels = {k: {"launches": c, "avg_ms": round(t / c, 4)} for k, (c, t) in sorted(prof.items())}
     line = {
-        "metric": "steps/s (SL-BDF2 step, 256^3 triquintic fp64)",
+        "metric": "steps/s (SL-BDF2 step, 256^3 triquintic fp64)" if args.storage == 64 else
+                  "steps/s (SL-BDF2 step, 256^3 triquintic, FP32-storage gathers)",
         "value": round(args.steps / (t1 - t0), 3),
         "unit": "steps/s",
         "n_gpus": 1,
@@ -128,23 +155,117 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.storage == 64 else "f64 (gathered fields stored f32)",
         "data": "synthetic (seeded solenoidal k^-5/3 perturbation on Poiseuille, projected)",
         "config": {"workload": f"SL-BDF2 channel {n}x{n}x{nz} order {args.order} "
                                f"Re_b {Re_b} dt+ {dtp}",
-                   "nx": n, "ny": n, "nz": nz, "interp_order": args.order, "dt": dt},
-        "roofline": {"bound": "hbm", "kernel": "bdf2_near_kernel<u,P>",
+                   "nx": n, "ny": n, "nz": nz, "interp_order": args.order, "dt": dt,
+                   "storage_bits": args.storage},
+        "roofline": {"bound": "hbm", "kernel": "sl_tile_kernel<u,P,FAR> (bdf2_far_u)",
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
-                     "bytes_per_launch": bytes_near, "avg_ms": round(avg_ms, 4)},
+                     "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": traffic,
+                     "bytes_per_launch": bytes_far, "avg_ms": round(avg_ms, 4)},
         "gather_GBps": round(gather_gbps, 1),
         "gather_frac_hbm": round(gather_gbps / PEAK_HBM_GBPS, 4),
         "max_div": div,
         "u_bulk": ubv,
         "kernels_ms": kernels,
+        "fp32_storage_rel_err_vs_fp64": fp32_err,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def large_box(args):
+    """1024 x 1024 x 512, Lx = 8 pi, Ly = 3 pi, tanh 1.8, Re_b = 20000, order 6,
+    k^-5/3 perturbation (u'rms 0.12), dt for trajectory CFL 4; fields resident.
+    Also: the isolated order-6 gather of u with feet displaced uniformly in
+    +-5 cells in x and y (seeded), z at the arrival point."""
+    import torch
+
+    from slchannel_amd import SLChannelFlow
+    from slchannel_amd.diagnostics import cfl_dt
+
+    nx, ny, nz = 1024, 1024, 512
+    cfg = {
+        "grid": {"nx": nx, "ny": ny, "nz": nz},
+        "domain": {"Lx": 8 * math.pi, "Ly": 3 * math.pi, "Lz": 2.0},
+        "flow": {"Re": 20000.0, "U_bulk": 1.0, "gamma": 1.8},
+        "sl": {"interp_order": args.order},
+    }
+    flow = SLChannelFlow(config=cfg, device="cuda:0")
+    flow.initialize(seed=3000, urms=0.12, kmin=1.0, kmax=32.0, spectrum="k-5/3")
+    dt = cfl_dt(flow, 4.0)
+    for _ in range(args.warmup):
+        flow.step_sl_bdf2(dt)
+    torch.cuda.synchronize()
+    plan = flow.plan
+    plan.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ub, _ = flow.step_sl_bdf2(dt)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = plan.profile_report()
+    plan.profile(False)
+    N_u = nx * ny * nz
+    cnt, tot = prof["bdf2_far_u"]
+    avg_ms = tot / cnt
+    achieved = 64.0 * N_u / (avg_ms * 1e-3) / 1e9
+    # isolated gather, feet = arrival + U(-5, 5) cells in x and y
+    g = torch.Generator(device="cuda").manual_seed(33)
+    ii = torch.arange(N_u, device="cuda", dtype=torch.int64)
+    i = (ii // (ny * nz)).double() + 1.0
+    j = ((ii // nz) % ny).double() + 1.0
+    xd = (i + 10.0 * torch.rand(N_u, generator=g, device="cuda", dtype=torch.float64) - 5.0) \
+        * flow.dx
+    yd = (j - 0.5 + 10.0 * torch.rand(N_u, generator=g, device="cuda", dtype=torch.float64)
+          - 5.0) * flow.dy
+    zd = flow.z_c[1:nz + 1].to("cuda").repeat(nx * ny)
+    del ii, i, j
+    out = torch.empty(N_u, dtype=torch.float64, device="cuda")
+    plan.gather("u", flow.u, xd, yd, zd, out)
+    torch.cuda.synchronize()
+    plan.profile(True)
+    for _ in range(3):
+        plan.gather("u", flow.u, xd, yd, zd, out)
+    gp = plan.profile_report()
+    plan.profile(False)
+    g_cnt, g_tot = gp["gather_u"]
+    gather_gbps = 40.0 * N_u / (g_tot / g_cnt * 1e-3) / 1e9
+    div = flow.divergence().abs().max().item()
+    line = {
+        "metric": "steps/s (SL-BDF2 step, 1024x1024x512 triquintic fp64)",
+        "value": round(args.steps / (t1 - t0), 4), "unit": "steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round((t1 - t0) * 1e3 / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded solenoidal k^-5/3 perturbation on Poiseuille, projected)",
+        "config": {"workload": "SL-BDF2 channel 1024x1024x512 order 6 Re_b 20000 CFL 4",
+                   "nx": nx, "ny": ny, "nz": nz, "interp_order": args.order, "dt": dt},
+        "roofline": {"bound": "hbm", "kernel": "sl_tile_kernel<u,P,FAR> (bdf2_far_u)",
+                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
+                     "bytes_per_launch": 64.0 * N_u, "avg_ms": round(avg_ms, 4)},
+        "gather_random_pm5_GBps": round(gather_gbps, 1),
+        "max_div": div, "u_bulk": ub.item(),
+        "kernels_ms": {k: {"launches": c, "avg_ms": round(t / c, 4)}
+                       for k, (c, t) in sorted(prof.items())},
+        "device_mem_GB": round(torch.cuda.max_memory_allocated() / 1e9, 1),
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def measured_traffic(n, nz, order):
+    """HBM bytes per launch of the roofline kernel from the committed PMC
+    summary (profiles/traffic_*.json, FETCH_SIZE + WRITE_SIZE scaled by the
+    copy-kernel calibration), if one exists for this workload; else None."""
+    path = os.path.join(REPO, "profiles", f"traffic_far_u_{n}x{n}x{nz}_o{order}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
 
 
 def cpu_baseline(flow, dt):

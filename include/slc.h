@@ -177,10 +177,23 @@ int slc_bulk_velocity(slc_plan* plan, const double* u, double* out, void* stream
 int slc_bulk_force(slc_plan* plan, double* u, double* v, double* w, double U_bulk, double dt,
                    double* out, void* stream);
 
-/* Tuning options (no counterpart in the reference).
- * SLC_OPT_GATHER_TILE: -1 = direct global-load gathers; 0..3 = LDS tile shape
- * (0: 4x8x32, 1: 2x8x32, 2: 2x4x64, 3: 4x4x32 arrival points). */
+/* Options.
+ * SLC_OPT_GATHER_TILE (no counterpart in the reference): -1 = direct
+ * global-load kernels; 0 = fused LDS-tiled step kernels (4x4x32 arrival
+ * points per tile, default); 1..3 = older standalone tiled gathers for
+ * slc_gather (1: 2x8x32, 2: 2x4x64, 3: 4x4x32).
+ * SLC_OPT_FIELD_STORAGE: 64 (default) or 32.  At 32 the gathers of
+ * slc_advect, slc_bdf1_predict and slc_bdf2_predict read float copies of the
+ * advected fields (made by the library in the same pass that forms U* or in
+ * one conversion pass), staged as float in LDS; Lagrange weights,
+ * accumulation, feet, the explicit xy-diffusion, z-solves and projection stay
+ * FP64.  It is the FP32-storage reading of the reference's "fp32_accum64"
+ * interp_dtype (semilag.py:340-356, semilag_triton.py:320-342), whose Triton
+ * gather stores and accumulates in fp32.  Selecting 32 allocates the float
+ * copies (2 x 4 B per field node) once.  Applies when
+ * SLC_OPT_GATHER_TILE >= 0; slc_gather always reads FP64. */
 #define SLC_OPT_GATHER_TILE 1
+#define SLC_OPT_FIELD_STORAGE 2
 int slc_set_option(slc_plan* plan, int option, int value);
 
 /* Opt-in launch timing (no counterpart in the reference, whose benchmarks

@@ -89,6 +89,9 @@ struct slc_plan {
     // opt-in launch timing: hipEvents recorded around each labelled launch
     bool prof = false;
     int gather_shape = 0;  // -1: direct global loads; >= 0: LDS tile shape
+    int storage = 64;      // 64: gathers read FP64 fields; 32: float copies (f32n / f32m)
+    float* f32n[3] = {nullptr, nullptr, nullptr};  // float copy of V^n per component
+    float* f32m[3] = {nullptr, nullptr, nullptr};  // float copy of V^{n-1} per component
     struct Rec {
         const char* label;
         hipEvent_t a, b;
@@ -460,7 +463,8 @@ int slc_gather(slc_plan* p, int comp, const double* field, int layout, const dou
     Timed t(p, kLabel[comp][4], stream);
     if (layout == 0 && p->gather_shape == 0) {
         TrajVel V{{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
-        HIP_OK(run_tile_step(p->g, comp, 4, p->at[comp], V, field, nullptr, 0.0, out,
+        HIP_OK(run_tile_step(p->g, comp, 4, p->at[comp], V, field, nullptr, nullptr, nullptr,
+                             0.0, out,
                              const_cast<double*>(xd), const_cast<double*>(yd),
                              const_cast<double*>(zd),
                              reinterpret_cast<unsigned long long*>(nclamp), S(stream)));
@@ -505,10 +509,16 @@ int slc_advect(slc_plan* p, const double* u, const double* v, const double* w, c
     const double* F[3] = {u, v, w};
     double* O[3] = {ustar, vstar, wstar};
     auto* ncl = reinterpret_cast<unsigned long long*>(nclamp);
+    const bool f32 = p->storage == 32 && p->gather_shape >= 0;
+    if (f32) {
+        Timed t(p, "to_f32", stream);
+        HIP_OK(run_to_f32(p->g, p->f32n, F, S(stream)));
+    }
     for (int c = 0; c < 3; ++c) {
         Timed t(p, kLabel[c][3], stream);
         if (p->gather_shape >= 0)
-            HIP_OK(run_tile_step(p->g, c, 0, p->at[c], V, F[c], nullptr, dt, O[c], nullptr,
+            HIP_OK(run_tile_step(p->g, c, 0, p->at[c], V, F[c], nullptr,
+                                 f32 ? p->f32n[c] : nullptr, nullptr, dt, O[c], nullptr,
                                  nullptr, nullptr, ncl, S(stream)));
         else
             HIP_OK(run_advect(p->g, c, V, F[c], dt, O[c], ncl, S(stream)));
@@ -526,10 +536,16 @@ int slc_bdf1_predict(slc_plan* p, const double* u, const double* v, const double
     const double* F[3] = {u, v, w};
     double* O[3] = {ustar, vstar, wstar};
     auto* ncl = reinterpret_cast<unsigned long long*>(nclamp);
+    const bool f32 = p->storage == 32 && p->gather_shape >= 0;
+    if (f32) {
+        Timed t(p, "to_f32", stream);
+        HIP_OK(run_to_f32(p->g, p->f32n, F, S(stream)));
+    }
     for (int c = 0; c < 3; ++c) {
         Timed t(p, kLabel[c][2], stream);
         if (p->gather_shape >= 0)
-            HIP_OK(run_tile_step(p->g, c, 1, p->at[c], V, F[c], nullptr, dt, O[c], nullptr,
+            HIP_OK(run_tile_step(p->g, c, 1, p->at[c], V, F[c], nullptr,
+                                 f32 ? p->f32n[c] : nullptr, nullptr, dt, O[c], nullptr,
                                  nullptr, nullptr, ncl, S(stream)));
         else
             HIP_OK(run_bdf1(p->g, c, V, F[c], dt, O[c], ncl, S(stream)));
@@ -550,17 +566,20 @@ int slc_bdf2_predict(slc_plan* p, const double* u, const double* v, const double
     double* O[3] = {ustar, vstar, wstar};
     auto* ncl = reinterpret_cast<unsigned long long*>(nclamp);
     if (p->gather_shape >= 0) {
+        const bool f32 = p->storage == 32;
         {
             Timed t(p, "ustar", stream);
-            HIP_OK(run_ustar(p->g, p->us, Fn, Fm, S(stream)));
+            HIP_OK(run_ustar(p->g, p->us, Fn, Fm, f32 ? p->f32n : nullptr,
+                             f32 ? p->f32m : nullptr, S(stream)));
         }
         TrajVel V{{p->us[0], p->us[1], p->us[2]}, {nullptr, nullptr, nullptr}};
         const long N = (long)p->nx * p->ny * p->nz;
         for (int c = 0; c < 3; ++c)
             for (int pass = 0; pass < 2; ++pass) {
                 Timed t(p, kLabel[c][pass], stream);
-                HIP_OK(run_tile_step(p->g, c, 2 + pass, p->at[c], V, Fn[c], Fm[c], dt, O[c],
-                                     p->nfoot, p->nfoot + N, p->nfoot + 2 * N, ncl,
+                HIP_OK(run_tile_step(p->g, c, 2 + pass, p->at[c], V, Fn[c], Fm[c],
+                                     f32 ? p->f32n[c] : nullptr, f32 ? p->f32m[c] : nullptr, dt,
+                                     O[c], p->nfoot, p->nfoot + N, p->nfoot + 2 * N, ncl,
                                      S(stream)));
             }
         return SLC_OK;
@@ -673,6 +692,26 @@ int slc_set_option(slc_plan* p, int option, int value) {
             if (value < -1 || value > 3) return fail(SLC_EINVAL, "gather tile shape %d", value);
             p->gather_shape = value;
             return SLC_OK;
+        case SLC_OPT_FIELD_STORAGE: {
+            if (value != 32 && value != 64)
+                return fail(SLC_EINVAL, "field storage must be 32 or 64 bits, got %d", value);
+            if (value == 32 && !p->f32n[0]) {
+                DeviceGuard dg(p->device);
+                const long n[3] = {(long)(p->nx + 1) * (p->ny + 2) * (p->nz + 2),
+                                   (long)(p->nx + 2) * (p->ny + 1) * (p->nz + 2),
+                                   (long)(p->nx + 2) * (p->ny + 2) * (p->nz + 1)};
+                for (int c = 0; c < 3; ++c) {
+                    void* a = nullptr;
+                    void* b = nullptr;
+                    if (int rc = dev_alloc(p, (size_t)n[c] * sizeof(float), &a)) return rc;
+                    if (int rc = dev_alloc(p, (size_t)n[c] * sizeof(float), &b)) return rc;
+                    p->f32n[c] = static_cast<float*>(a);
+                    p->f32m[c] = static_cast<float*>(b);
+                }
+            }
+            p->storage = value;
+            return SLC_OK;
+        }
         default:
             return fail(SLC_EINVAL, "unknown option %d", option);
     }

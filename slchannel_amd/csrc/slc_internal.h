@@ -48,14 +48,18 @@ struct ArrivalTabs {
 
 // LDS-tiled step kernels (slc_step.hip); mode: 0 advect, 1 bdf1, 2 far,
 // 3 near, 4 gather (feet given)
+// Fn32 / Fm32: float copies of F / Fm to gather from (FP32 storage), or null.
 hipError_t run_tile_step(const Grid& g, int comp, int mode, const ArrivalTabs& A,
-                         const TrajVel& V, const double* F, const double* Fm, double dt,
-                         double* out, double* fx, double* fy, double* fz,
-                         unsigned long long* ncl, hipStream_t s);
+                         const TrajVel& V, const double* F, const double* Fm, const float* Fn32,
+                         const float* Fm32, double dt, double* out, double* fx, double* fy,
+                         double* fz, unsigned long long* ncl, hipStream_t s);
 hipError_t run_arrival_tabs(const Grid& g, int comp, double* xdx, double* ydy, int* cell,
                             hipStream_t s);
 hipError_t run_ustar(const Grid& g, double* const o[3], const double* const a[3],
-                     const double* const b[3], hipStream_t s);
+                     const double* const b[3], float* const fa[3], float* const fb[3],
+                     hipStream_t s);
+hipError_t run_to_f32(const Grid& g, float* const o[3], const double* const a[3],
+                      hipStream_t s);
 
 // field operators (slc_ops.hip)
 hipError_t run_fill_u64(unsigned long long* p, long n, unsigned long long v, hipStream_t s);

@@ -117,15 +117,20 @@ __device__ inline void make_st(const Grid& g, double x, double y, double z, Foot
     stencil_base<C, P>(g, x, y, f.z, f.bx, f.by, f.bk, f.tx, f.ty);
 }
 
-template <int C, int P, int MODE, int TI, int TJ, int TK, int CAP>
+// T: storage type of the gathered field (double, or float for the FP32-storage
+// variant: values are staged and read as float, weights and accumulation stay
+// FP64).  F / Fm are always the FP64 fields (explicit xy-diffusion);
+// Sn / Sm are the gathered copies (== F / Fm when T is double).
+template <typename T, int C, int P, int MODE, int TI, int TJ, int TK, int LDS_BYTES>
 __global__ __launch_bounds__(kSB) void sl_tile_kernel(
     Grid g, ArrivalTabs A, TrajVel V, const double* __restrict__ F,
-    const double* __restrict__ Fm, double dt, double* __restrict__ out, Feet ft,
-    unsigned long long* nclamp) {
+    const double* __restrict__ Fm, const T* __restrict__ Sn, const T* __restrict__ Sm,
+    double dt, double* __restrict__ out, Feet ft, unsigned long long* nclamp) {
     constexpr int NPT = TI * TJ * TK;
     static_assert(NPT % kSB == 0, "tile must be a multiple of the block");
     constexpr int PT = NPT / kSB;
-    __shared__ double lds[CAP];
+    constexpr int CAP = LDS_BYTES / (int)sizeof(T);
+    __shared__ T lds[CAP];
     __shared__ int sh[6];
     __shared__ int s_cnt;
     const int nzi = Comp<C>::NZI(g);
@@ -203,7 +208,7 @@ __global__ __launch_bounds__(kSB) void sl_tile_kernel(
     }
 
     // ---- stage the tile's stencil box in LDS -------------------------------
-    const double* __restrict__ S = (MODE == MODE_FAR) ? Fm : F;
+    const T* __restrict__ S = (MODE == MODE_FAR) ? Sm : Sn;
     const SBox b = s_block_box(l, sh);
     const int BX = b.xmax - b.xmin + P, BY = b.ymax - b.ymin + P, BZ = b.kmax - b.kmin + P;
     const bool fits = b.xmin <= b.xmax && (long)BX * BY * BZ <= CAP;
@@ -249,10 +254,10 @@ __global__ __launch_bounds__(kSB) void sl_tile_kernel(
                 double ai = 0.0;
 #pragma unroll
                 for (int j = 0; j < P; ++j) {
-                    const double* col = lds + ((ox + i) * BY + (oy + j)) * BZ + oz;
+                    const T* col = lds + ((ox + i) * BY + (oy + j)) * BZ + oz;
                     double sj = 0.0;
 #pragma unroll
-                    for (int k = 0; k < P; ++k) sj = fma(wz[k], col[k], sj);
+                    for (int k = 0; k < P; ++k) sj = fma(wz[k], (double)col[k], sj);
                     ai = fma(wy[j], sj, ai);
                 }
                 acc = fma(wx[i], ai, acc);
@@ -265,10 +270,10 @@ __global__ __launch_bounds__(kSB) void sl_tile_kernel(
 #pragma unroll
                 for (int j = 0; j < P; ++j) {
                     const int gy = wrap_idx(f.by + j, g.ny);
-                    const double* col = S + node_addr<C>(g, gx, gy, f.bk);
+                    const T* col = S + node_addr<C>(g, gx, gy, f.bk);
                     double sj = 0.0;
 #pragma unroll
-                    for (int k = 0; k < P; ++k) sj = fma(wz[k], col[k], sj);
+                    for (int k = 0; k < P; ++k) sj = fma(wz[k], (double)col[k], sj);
                     ai = fma(wy[j], sj, ai);
                 }
                 acc = fma(uniform_weight_m<P>(f.tx, i), ai, acc);
@@ -341,74 +346,144 @@ hipError_t run_arrival_tabs(const Grid& g, int comp, double* xdx, double* ydy, i
 
 // U* = 2a - b elementwise over three whole ghost-shaped fields
 // (solver.py:866-868: the same single rounding).
+// With fa/fb non-null also writes float copies of a and b (FP32 storage).
+struct F32x3 {
+    float* p[3];
+};
+
 __global__ void ustar_kernel(double* __restrict__ o0, double* __restrict__ o1,
                              double* __restrict__ o2, const double* __restrict__ a0,
                              const double* __restrict__ a1, const double* __restrict__ a2,
                              const double* __restrict__ b0, const double* __restrict__ b1,
-                             const double* __restrict__ b2, long n0, long n1, long n2) {
+                             const double* __restrict__ b2, long n0, long n1, long n2,
+                             F32x3 fa, F32x3 fb) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n0) o0[i] = 2.0 * a0[i] - b0[i];
-    if (i < n1) o1[i] = 2.0 * a1[i] - b1[i];
-    if (i < n2) o2[i] = 2.0 * a2[i] - b2[i];
+    const bool f32 = fa.p[0] != nullptr;
+    if (i < n0) {
+        const double a = a0[i], b = b0[i];
+        o0[i] = 2.0 * a - b;
+        if (f32) { fa.p[0][i] = (float)a; fb.p[0][i] = (float)b; }
+    }
+    if (i < n1) {
+        const double a = a1[i], b = b1[i];
+        o1[i] = 2.0 * a - b;
+        if (f32) { fa.p[1][i] = (float)a; fb.p[1][i] = (float)b; }
+    }
+    if (i < n2) {
+        const double a = a2[i], b = b2[i];
+        o2[i] = 2.0 * a - b;
+        if (f32) { fa.p[2][i] = (float)a; fb.p[2][i] = (float)b; }
+    }
+}
+
+static inline void field_sizes(const Grid& g, long n[3]) {
+    n[0] = (long)(g.nx + 1) * (g.ny + 2) * (g.nz + 2);
+    n[1] = (long)(g.nx + 2) * (g.ny + 1) * (g.nz + 2);
+    n[2] = (long)(g.nx + 2) * (g.ny + 2) * (g.nz + 1);
 }
 
 hipError_t run_ustar(const Grid& g, double* const o[3], const double* const a[3],
-                     const double* const b[3], hipStream_t s) {
-    const long n0 = (long)(g.nx + 1) * (g.ny + 2) * (g.nz + 2);
-    const long n1 = (long)(g.nx + 2) * (g.ny + 1) * (g.nz + 2);
-    const long n2 = (long)(g.nx + 2) * (g.ny + 2) * (g.nz + 1);
-    long nmax = n0 > n1 ? n0 : n1;
-    nmax = nmax > n2 ? nmax : n2;
+                     const double* const b[3], float* const fa[3], float* const fb[3],
+                     hipStream_t s) {
+    long n[3];
+    field_sizes(g, n);
+    long nmax = n[0] > n[1] ? n[0] : n[1];
+    nmax = nmax > n[2] ? nmax : n[2];
+    F32x3 A{{fa ? fa[0] : nullptr, fa ? fa[1] : nullptr, fa ? fa[2] : nullptr}};
+    F32x3 B{{fb ? fb[0] : nullptr, fb ? fb[1] : nullptr, fb ? fb[2] : nullptr}};
     hipLaunchKernelGGL(ustar_kernel, dim3((unsigned)((nmax + 255) / 256)), dim3(256), 0, s, o[0],
-                       o[1], o[2], a[0], a[1], a[2], b[0], b[1], b[2], n0, n1, n2);
+                       o[1], o[2], a[0], a[1], a[2], b[0], b[1], b[2], n[0], n[1], n[2], A, B);
+    return hipGetLastError();
+}
+
+// float copies of three ghost-shaped fields (FP32 storage for advect / BDF1)
+__global__ void to_f32_kernel(float* __restrict__ o0, float* __restrict__ o1,
+                              float* __restrict__ o2, const double* __restrict__ a0,
+                              const double* __restrict__ a1, const double* __restrict__ a2,
+                              long n0, long n1, long n2) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n0) o0[i] = (float)a0[i];
+    if (i < n1) o1[i] = (float)a1[i];
+    if (i < n2) o2[i] = (float)a2[i];
+}
+
+hipError_t run_to_f32(const Grid& g, float* const o[3], const double* const a[3],
+                      hipStream_t s) {
+    long n[3];
+    field_sizes(g, n);
+    long nmax = n[0] > n[1] ? n[0] : n[1];
+    nmax = nmax > n[2] ? nmax : n[2];
+    hipLaunchKernelGGL(to_f32_kernel, dim3((unsigned)((nmax + 255) / 256)), dim3(256), 0, s, o[0],
+                       o[1], o[2], a[0], a[1], a[2], n[0], n[1], n[2]);
     return hipGetLastError();
 }
 
 // tile shape of the step kernels: 4 x 4 x 32 arrival points, 40 KB of LDS
-constexpr int kTI = 4, kTJ = 4, kTK = 32, kCAP = 5120;
+constexpr int kTI = 4, kTJ = 4, kTK = 32, kLDS = 40960;
 
-template <int C, int P, int MODE>
-static void launch_mode(const Grid& g, const ArrivalTabs& A, const TrajVel& V, const double* F,
-                        const double* Fm, double dt, double* out, const Feet& ft,
-                        unsigned long long* ncl, hipStream_t s) {
+struct StepArgs {
+    const double* F;
+    const double* Fm;
+    const void* Sn;  // gathered copies (double or float)
+    const void* Sm;
+    double dt;
+    double* out;
+    Feet ft;
+    unsigned long long* ncl;
+};
+
+template <typename T, int C, int P, int MODE>
+static void launch_mode(const Grid& g, const ArrivalTabs& A, const TrajVel& V,
+                        const StepArgs& a, hipStream_t s) {
     const long nzi = (C == 2) ? g.nz - 1 : g.nz;
     const unsigned tiles = (unsigned)(((long)(g.nx + kTI - 1) / kTI) *
                                       ((g.ny + kTJ - 1) / kTJ) * ((nzi + kTK - 1) / kTK));
     if (tiles == 0) return;
-    hipLaunchKernelGGL((sl_tile_kernel<C, P, MODE, kTI, kTJ, kTK, kCAP>), dim3(tiles), dim3(kSB),
-                       0, s, g, A, V, F, Fm, dt, out, ft, ncl);
+    hipLaunchKernelGGL((sl_tile_kernel<T, C, P, MODE, kTI, kTJ, kTK, kLDS>), dim3(tiles),
+                       dim3(kSB), 0, s, g, A, V, a.F, a.Fm, (const T*)a.Sn, (const T*)a.Sm, a.dt,
+                       a.out, a.ft, a.ncl);
 }
 
-template <int C, int MODE>
-static void launch_cp(const Grid& g, const ArrivalTabs& A, const TrajVel& V, const double* F,
-                      const double* Fm, double dt, double* out, const Feet& ft,
-                      unsigned long long* ncl, hipStream_t s) {
-    if (g.order == 4) launch_mode<C, 4, MODE>(g, A, V, F, Fm, dt, out, ft, ncl, s);
-    else launch_mode<C, 6, MODE>(g, A, V, F, Fm, dt, out, ft, ncl, s);
+template <typename T, int C, int MODE>
+static void launch_cp(const Grid& g, const ArrivalTabs& A, const TrajVel& V, const StepArgs& a,
+                      hipStream_t s) {
+    if (g.order == 4) launch_mode<T, C, 4, MODE>(g, A, V, a, s);
+    else launch_mode<T, C, 6, MODE>(g, A, V, a, s);
 }
 
-template <int MODE>
+template <typename T, int MODE>
 static void launch_c(const Grid& g, int comp, const ArrivalTabs& A, const TrajVel& V,
-                     const double* F, const double* Fm, double dt, double* out, const Feet& ft,
-                     unsigned long long* ncl, hipStream_t s) {
+                     const StepArgs& a, hipStream_t s) {
     switch (comp) {
-        case 0: launch_cp<0, MODE>(g, A, V, F, Fm, dt, out, ft, ncl, s); break;
-        case 1: launch_cp<1, MODE>(g, A, V, F, Fm, dt, out, ft, ncl, s); break;
-        default: launch_cp<2, MODE>(g, A, V, F, Fm, dt, out, ft, ncl, s); break;
+        case 0: launch_cp<T, 0, MODE>(g, A, V, a, s); break;
+        case 1: launch_cp<T, 1, MODE>(g, A, V, a, s); break;
+        default: launch_cp<T, 2, MODE>(g, A, V, a, s); break;
+    }
+}
+
+template <typename T>
+static void launch_t(const Grid& g, int comp, int mode, const ArrivalTabs& A, const TrajVel& V,
+                     const StepArgs& a, hipStream_t s) {
+    switch (mode) {
+        case MODE_ADVECT: launch_c<T, MODE_ADVECT>(g, comp, A, V, a, s); break;
+        case MODE_BDF1: launch_c<T, MODE_BDF1>(g, comp, A, V, a, s); break;
+        case MODE_FAR: launch_c<T, MODE_FAR>(g, comp, A, V, a, s); break;
+        case MODE_NEAR: launch_c<T, MODE_NEAR>(g, comp, A, V, a, s); break;
+        default: launch_c<T, MODE_GATHER>(g, comp, A, V, a, s); break;
     }
 }
 
 hipError_t run_tile_step(const Grid& g, int comp, int mode, const ArrivalTabs& A,
-                         const TrajVel& V, const double* F, const double* Fm, double dt,
-                         double* out, double* fx, double* fy, double* fz,
-                         unsigned long long* ncl, hipStream_t s) {
+                         const TrajVel& V, const double* F, const double* Fm, const float* Fn32,
+                         const float* Fm32, double dt, double* out, double* fx, double* fy,
+                         double* fz, unsigned long long* ncl, hipStream_t s) {
     const Feet ft{fx, fy, fz};
-    switch (mode) {
-        case MODE_ADVECT: launch_c<MODE_ADVECT>(g, comp, A, V, F, Fm, dt, out, ft, ncl, s); break;
-        case MODE_BDF1: launch_c<MODE_BDF1>(g, comp, A, V, F, Fm, dt, out, ft, ncl, s); break;
-        case MODE_FAR: launch_c<MODE_FAR>(g, comp, A, V, F, Fm, dt, out, ft, ncl, s); break;
-        case MODE_NEAR: launch_c<MODE_NEAR>(g, comp, A, V, F, Fm, dt, out, ft, ncl, s); break;
-        default: launch_c<MODE_GATHER>(g, comp, A, V, F, Fm, dt, out, ft, ncl, s); break;
+    if (Fn32 || Fm32) {
+        StepArgs a{F, Fm, Fn32, Fm32, dt, out, ft, ncl};
+        launch_t<float>(g, comp, mode, A, V, a, s);
+    } else {
+        StepArgs a{F, Fm, F, Fm, dt, out, ft, ncl};
+        launch_t<double>(g, comp, mode, A, V, a, s);
     }
     return hipGetLastError();
 }

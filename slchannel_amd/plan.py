@@ -97,6 +97,7 @@ class HipPlan:
         return out.value
 
     OPT_GATHER_TILE = 1
+    OPT_FIELD_STORAGE = 2
 
     def set_option(self, option, value):
         """Tuning knob (include/slc.h: SLC_OPT_*)."""

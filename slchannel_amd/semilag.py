@@ -105,6 +105,10 @@ class SLAdvector:
                            n_traj_iters=n_traj_iters, top_wall_bc_type=top_wall_bc_type,
                            device=device)
         self.plan = plan
+        # fp32_accum64: gathered fields stored as float, weights/accumulation
+        # and everything else FP64 (the reference runs the whole interpolation
+        # in fp32 there, semilag.py:249-256)
+        plan.set_option(HipPlan.OPT_FIELD_STORAGE, 32 if interp_dtype == "fp32_accum64" else 64)
         self.device = plan.device
         self.z_lo, self.z_hi = plan.z_lo, plan.z_hi
         self._tanh_g = plan.tanh_g

@@ -414,3 +414,97 @@ def test_fast_z_locate_equals_analytic(check, stretching, gamma, nz):
         off = ~tie
         check(f"{comp}: analytic == oracle off ties", np.array_equal(ex[off], ref[off]),
               f"mismatches={int((ex[off] != ref[off]).sum())} ties={int(tie.sum())}")
+
+
+def _lebesgue(order, n=2001):
+    """1-D Lebesgue constant of the order-point Lagrange stencil on uniform
+    nodes over its central interval (where the gathers evaluate it)."""
+    nodes = np.arange(order) - (order // 2 - 1)
+    t = np.linspace(0.0, 1.0, n)
+    tot = np.zeros_like(t)
+    for m in range(order):
+        lm = np.ones_like(t)
+        for l in range(order):
+            if l != m:
+                lm *= (t - nodes[l]) / (nodes[m] - nodes[l])
+        tot += np.abs(lm)
+    return float(tot.max())
+
+
+@pytest.mark.parametrize("order", [4, 6])
+def test_fp32_storage_error_per_field(check, order):
+    """FP32-storage gathers (float fields, FP64 weights/accumulation/feet)
+    against the FP64 path: same clamp counts (the feet do not change), and
+    per field |d| <= 2^-24 * Lambda^3 * max|F| (times 4/3 + 1/3 for the BDF2
+    predictor), Lambda the 1-D Lebesgue constant of the stencil (x1.1 for the
+    stretched z nodes).  The error must also be non-zero: the float copies
+    are really read."""
+    c = _random_case(40, 36, 48, order, "symmetric", seed=71, cfl=3.0)
+    plan = make_plan(c, nu=1e-3)
+    f = [cuda(a) for a in c["fields"]]
+    m = [cuda(a) for a in c["mids"]]
+    dt = c["dt"]
+    lam = _lebesgue(order)
+    base = 2.0**-24 * lam**3 * 1.1
+    res = {}
+    for bits in (64, 32):
+        plan.set_option(plan.OPT_FIELD_STORAGE, bits)
+        outs = {}
+        for name in ("advect", "bdf1", "bdf2"):
+            o = [torch.zeros(plan.shapes[k], dtype=torch.float64, device="cuda") for k in "uvw"]
+            ncl = torch.zeros(1, dtype=torch.int64, device="cuda")
+            if name == "advect":
+                plan.advect(*f, *m, dt, *o, ncl)
+            elif name == "bdf1":
+                plan.bdf1_predict(*f, dt, *o, ncl)
+            else:
+                plan.bdf2_predict(*f, *m, dt, *o, ncl)
+            outs[name] = (o, int(ncl.item()))
+        res[bits] = outs
+    plan.set_option(plan.OPT_FIELD_STORAGE, 64)
+    for name in ("advect", "bdf1", "bdf2"):
+        (a, na), (b, nb) = res[32][name], res[64][name]
+        scale = base * ((4.0 / 3.0 + 1.0 / 3.0) if name == "bdf2" else 1.0)
+        srcs = c["fields"] if name != "bdf2" else [np.maximum(np.abs(x), np.abs(y)) for x, y in
+                                                    zip(c["fields"], c["mids"])]
+        for comp, x, y, s in zip("uvw", a, b, srcs):
+            d = (x - y).abs().max().item()
+            bound = scale * float(np.abs(s).max())
+            check(f"{name} {comp}: fp32 storage vs fp64", 0.0 < d <= bound,
+                  f"max|d|={d:.2e} bound={bound:.2e} rel={d / float(np.abs(s).max()):.1e}")
+        check(f"{name}: clamp counts unchanged", na == nb, f"{na} vs {nb}")
+
+
+def test_fp32_storage_step_vs_oracle(check):
+    """interp_dtype fp32_accum64 (FP32-storage gathers) over 8 BDF1/BDF2
+    steps of the seeded order-6 case against the FP64 oracle: per-field
+    error stated, below 1e-6 relative; the FP64 projection still gives
+    max|div| < 1e-11 and u_bulk = 1.0 every step."""
+    from slchannel_amd import SLChannelFlow
+    from slchannel_amd.diagnostics import cfl_dt
+
+    cfg = _config(32, order=6, nz=24)
+    cfg["sl"]["interp_dtype"] = "fp32_accum64"
+    flow = SLChannelFlow(config=cfg)
+    flow.initialize(seed=1001, urms=0.1, kmax=6.0)
+    co = orc.ChannelOracle(flow.nx, flow.ny, flow.nz, flow.Lx, flow.Ly, flow.Lz, flow.gamma,
+                           2800.0, order=6, z_f=flow.z_f.numpy(), z_c=flow.z_c.numpy(),
+                           tanh_g=flow.plan.tanh_g)
+    co.set_fields(flow.u.cpu().numpy(), flow.v.cpu().numpy(), flow.w.cpu().numpy(),
+                  flow.p.cpu().numpy())
+    dt = cfl_dt(flow, 3.0)
+    errs = {k: 0.0 for k in "uvwp"}
+    for n in range(8):
+        ub, _ = flow.step_sl_bdf2(dt)
+        co.step_sl_bdf2(dt)
+        for k, a, b in (("u", flow.u, co.u), ("v", flow.v, co.v), ("w", flow.w, co.w),
+                        ("p", flow.p, co.p)):
+            errs[k] = max(errs[k], rel_err(a, b))
+        md = flow.divergence().abs().max().item()
+        check(f"step {n + 1} max|div|", md < 1e-11, f"{md:.2e}")
+        check(f"step {n + 1} u_bulk", abs(ub.item() - 1.0) < 1e-14, f"{ub.item()!r}")
+    for k in "uvw":
+        check(f"{k}: fp32 storage vs fp64 oracle over 8 steps", 0.0 < errs[k] < 1e-6,
+              f"max rel={errs[k]:.2e}")
+    check("p: fp32 storage vs fp64 oracle over 8 steps", errs["p"] < 1e-5,
+          f"max rel={errs['p']:.2e}")
