@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--nz", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 1 GiB device-to-device copy 3 times (a kernel of "
+                         "known traffic, to scale the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--storage", type=int, default=64, choices=(64, 32),
                     help="bits of the gathered field copies: 64 (default) or 32 "
                          "(FP32-storage variant; adds an FP64 twin run and reports the "
@@ -64,6 +67,13 @@ def main():
 
     from slchannel_amd import SLChannelFlow
 
+    if args.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if args.config == 3:
         return large_box(args)
     n, nz = args.n, args.nz or args.n

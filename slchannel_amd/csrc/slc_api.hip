@@ -14,16 +14,14 @@
 #include <string>
 #include <vector>
 
-#include "../../include/slc.h"
-#include "slc_internal.h"
+#include "slc_plan.h"
 
 using namespace slc;
 
-namespace {
+namespace slc {
 
 thread_local std::string g_err;
 
-int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 int fail(int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
@@ -33,78 +31,6 @@ int fail(int code, const char* fmt, ...) {
     g_err = buf;
     return code;
 }
-
-#define HIP_OK(expr)                                                                       \
-    do {                                                                                   \
-        hipError_t e_ = (expr);                                                            \
-        if (e_ != hipSuccess)                                                              \
-            return fail(SLC_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                        __FILE__, __LINE__);                                               \
-    } while (0)
-
-#define FFT_OK(expr)                                                                   \
-    do {                                                                               \
-        rocfft_status s_ = (expr);                                                     \
-        if (s_ != rocfft_status_success)                                               \
-            return fail(SLC_EFFT, "%s failed: rocfft status %d", #expr, (int)s_);     \
-    } while (0)
-
-std::once_flag g_fft_once;
-
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev) {
-        if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
-        else prev = -1;
-    }
-    ~DeviceGuard() {
-        if (prev >= 0) (void)hipSetDevice(prev);
-    }
-};
-
-}  // namespace
-
-struct slc_plan {
-    int device = 0;
-    Grid g{};
-    PoissonTabs pt{};
-    int nx = 0, ny = 0, nz = 0, nky = 0;
-    double total_volume = 0.0;
-    std::vector<void*> allocs;
-    int64_t bytes = 0;
-    const double* d_dzc = nullptr;
-    const double* d_dzf = nullptr;
-    const double* d_cv = nullptr;
-    double* planes = nullptr;   // (nz, nx, ny) real
-    double* dhat = nullptr;     // (nz, nx, nky) complex
-    double* cpw = nullptr;      // (nz, nx, nky) Thomas c'
-    double* ztab = nullptr;     // per component: 3 * (nz + 2) z-solve factor
-    double zkey[3][2] = {};     // (dt, theta) the factor of each component was built for
-    bool zvalid[3] = {false, false, false};
-    double* partials = nullptr;
-    int max_partials = 0;
-    double* us[3] = {nullptr, nullptr, nullptr};  // U* = 2V^n - V^{n-1} (ghost-shaped)
-    double* nfoot = nullptr;                      // near feet of one component (3 x N)
-    ArrivalTabs at[3] = {};
-    // opt-in launch timing: hipEvents recorded around each labelled launch
-    bool prof = false;
-    int gather_shape = 0;  // -1: direct global loads; >= 0: LDS tile shape
-    int storage = 64;      // 64: gathers read FP64 fields; 32: float copies (f32n / f32m)
-    float* f32n[3] = {nullptr, nullptr, nullptr};  // float copy of V^n per component
-    float* f32m[3] = {nullptr, nullptr, nullptr};  // float copy of V^{n-1} per component
-    struct Rec {
-        const char* label;
-        hipEvent_t a, b;
-    };
-    std::vector<Rec> recs;
-    std::vector<hipEvent_t> pool;
-    rocfft_plan fwd = nullptr, inv = nullptr;
-    rocfft_execution_info info = nullptr;
-    void* fft_work = nullptr;
-    size_t fft_work_bytes = 0;
-};
-
-namespace {
 
 int dev_alloc(slc_plan* p, size_t bytes, void** out) {
     void* ptr = nullptr;
@@ -116,6 +42,12 @@ int dev_alloc(slc_plan* p, size_t bytes, void** out) {
     *out = ptr;
     return SLC_OK;
 }
+
+}  // namespace slc
+
+namespace {
+
+std::once_flag g_fft_once;
 
 int upload(slc_plan* p, const std::vector<double>& h, const double** out) {
     void* d = nullptr;
@@ -229,29 +161,7 @@ int poisson_planes(slc_plan* p, void* stream) {
 
 }  // namespace
 
-extern "C" {
-
-int slc_version(void) { return 100; }
-
-const char* slc_last_error(void) { return g_err.c_str(); }
-
-int slc_plan_destroy(slc_plan* p) {
-    if (!p) return SLC_OK;
-    DeviceGuard dg(p->device);
-    if (p->fwd) rocfft_plan_destroy(p->fwd);
-    if (p->inv) rocfft_plan_destroy(p->inv);
-    if (p->info) rocfft_execution_info_destroy(p->info);
-    for (auto& r : p->recs) {
-        (void)hipEventDestroy(r.a);
-        (void)hipEventDestroy(r.b);
-    }
-    for (hipEvent_t e : p->pool) (void)hipEventDestroy(e);
-    for (void* a : p->allocs) (void)hipFree(a);
-    delete p;
-    return SLC_OK;
-}
-
-int slc_plan_create(const slc_grid_desc* d, int device, slc_plan** out) {
+int slc::plan_create(const slc_grid_desc* d, int device, bool spectral, slc_plan** out) {
     if (!d || !out) return fail(SLC_EINVAL, "null argument");
     *out = nullptr;
     if (d->nx < 1 || d->ny < 2 || d->nz < 2)
@@ -348,12 +258,14 @@ int slc_plan_create(const slc_grid_desc* d, int device, slc_plan** out) {
 
     const long nplane = (long)nx * ny, nmodes = (long)nx * p->nky;
     void* ptr = nullptr;
-    if ((rc = dev_alloc(p, sizeof(double) * nplane * nz, &ptr))) { slc_plan_destroy(p); return rc; }
-    p->planes = (double*)ptr;
-    if ((rc = dev_alloc(p, 2 * sizeof(double) * nmodes * nz, &ptr))) { slc_plan_destroy(p); return rc; }
-    p->dhat = (double*)ptr;
-    if ((rc = dev_alloc(p, sizeof(double) * nmodes * nz, &ptr))) { slc_plan_destroy(p); return rc; }
-    p->cpw = (double*)ptr;
+    if (spectral) {
+        if ((rc = dev_alloc(p, sizeof(double) * nplane * nz, &ptr))) { slc_plan_destroy(p); return rc; }
+        p->planes = (double*)ptr;
+        if ((rc = dev_alloc(p, 2 * sizeof(double) * nmodes * nz, &ptr))) { slc_plan_destroy(p); return rc; }
+        p->dhat = (double*)ptr;
+        if ((rc = dev_alloc(p, sizeof(double) * nmodes * nz, &ptr))) { slc_plan_destroy(p); return rc; }
+        p->cpw = (double*)ptr;
+    }
     if ((rc = dev_alloc(p, sizeof(double) * 9 * (nz + 2), &ptr))) { slc_plan_destroy(p); return rc; }
     p->ztab = (double*)ptr;
     {
@@ -391,6 +303,10 @@ int slc_plan_create(const slc_grid_desc* d, int device, slc_plan** out) {
     if ((rc = dev_alloc(p, sizeof(double) * p->max_partials, &ptr))) { slc_plan_destroy(p); return rc; }
     p->partials = (double*)ptr;
 
+    if (!spectral) {
+        *out = p;
+        return SLC_OK;
+    }
     // rocFFT: nz batched 2-D transforms of contiguous (nx, ny) planes; the
     // halved (Hermitian) dimension is y, as torch.fft.rfft2 over dims (0, 1).
     size_t lengths[2] = {(size_t)ny, (size_t)nx};  // fastest first
@@ -419,6 +335,32 @@ int slc_plan_create(const slc_grid_desc* d, int device, slc_plan** out) {
     }
     *out = p;
     return SLC_OK;
+}
+
+extern "C" {
+
+int slc_version(void) { return 100; }
+
+const char* slc_last_error(void) { return g_err.c_str(); }
+
+int slc_plan_destroy(slc_plan* p) {
+    if (!p) return SLC_OK;
+    DeviceGuard dg(p->device);
+    if (p->fwd) rocfft_plan_destroy(p->fwd);
+    if (p->inv) rocfft_plan_destroy(p->inv);
+    if (p->info) rocfft_execution_info_destroy(p->info);
+    for (auto& r : p->recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (hipEvent_t e : p->pool) (void)hipEventDestroy(e);
+    for (void* a : p->allocs) (void)hipFree(a);
+    delete p;
+    return SLC_OK;
+}
+
+int slc_plan_create(const slc_grid_desc* d, int device, slc_plan** out) {
+    return plan_create(d, device, true, out);
 }
 
 int slc_plan_workspace_bytes(const slc_plan* p, int64_t* out) {

@@ -66,6 +66,8 @@ hipError_t run_fill_u64(unsigned long long* p, long n, unsigned long long v, hip
 hipError_t run_fill_f64(double* p, long n, double v, hipStream_t s);
 hipError_t run_bc(const Grid& g, double* u, double* v, double* w, hipStream_t s);
 hipError_t run_bc_p(const Grid& g, double* p, hipStream_t s);
+hipError_t run_zfactor(const Grid& g, int comp, double dt, double theta, const double* dz_c,
+                       const double* dz_f, double* tab, hipStream_t s);
 hipError_t run_zsolve(const Grid& g, int comp, double* f, double dt, double theta,
                       const double* dz_c, const double* dz_f, double* tab, bool refactor,
                       hipStream_t s);

@@ -1,0 +1,479 @@
+// slc_ops_dev.h -- device bodies of the step's field operators, shared by the
+// single-run kernels (slc_ops.hip) and the batched ensemble kernels
+// (slc_ens_kernels.hip), so both compute the very same arithmetic:
+//   ghost/boundary fill            solver.py:70-107, projection.py:180-195
+//   theta-method z-diffusion       operators.py:382-630 (Thomas, shared factor)
+//   divergence (+transpose)        utils.py:166-180
+//   per-wavenumber Poisson Thomas  projection.py:133-247 (coefficients on the fly)
+//   gradient projection            projection.py:250-272
+//   bulk-flux constraint           solver.py:694-723, utils.py:147-163
+// Each body takes its work coordinates explicitly (block or thread index
+// within one channel) instead of reading blockIdx.
+#pragma once
+
+#include "slc_internal.h"
+
+namespace slc {
+
+// ---------------------------------------------------------------------------
+// ghost fill
+// ---------------------------------------------------------------------------
+// The reference assigns ghost slabs in a fixed order (x, then y, then z), so
+// every ghost's final value is +-(one interior value).  Each thread writes one
+// ghost entry from its interior source; interiors are only read, so there is
+// no ordering hazard.  Entries are enumerated as
+//   S1: whole x-ghost planes          (contiguous)
+//   S2: y-ghost rows of interior x    (contiguous along z)
+//   S3: the two z ends of interior columns.
+// Field kinds: 0 u, 1 v, 2 w, 3 p.
+
+struct GhostShape {
+    int NI, NJ, NK;
+    int nxg, xg0, xg1;  // x-ghost planes
+    int nyg, yg0, yg1;  // y-ghost rows
+    long S1, S2, S3;
+};
+
+__host__ __device__ inline GhostShape ghost_shape(int kind, int nx, int ny, int nz) {
+    GhostShape s;
+    s.NI = (kind == 0) ? nx + 1 : nx + 2;
+    s.NJ = (kind == 1) ? ny + 1 : ny + 2;
+    s.NK = (kind == 2) ? nz + 1 : nz + 2;
+    if (kind == 0) { s.nxg = 1; s.xg0 = 0; s.xg1 = 0; }
+    else { s.nxg = 2; s.xg0 = 0; s.xg1 = nx + 1; }
+    if (kind == 1) { s.nyg = 1; s.yg0 = 0; s.yg1 = 0; }
+    else { s.nyg = 2; s.yg0 = 0; s.yg1 = ny + 1; }
+    s.S1 = (long)s.nxg * s.NJ * s.NK;
+    s.S2 = (long)nx * s.nyg * s.NK;
+    s.S3 = (long)nx * ny * 2;
+    return s;
+}
+
+__host__ __device__ inline long ghost_count(int kind, int nx, int ny, int nz) {
+    const GhostShape s = ghost_shape(kind, nx, ny, nz);
+    return s.S1 + s.S2 + s.S3;
+}
+
+__device__ inline void ghost_value(int kind, const Grid& g, const double* F, const GhostShape& s,
+                                   int I, int J, int K, double* out_val) {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    int is = I, js = J, ks = K;
+    double sg = 1.0;
+    bool zero = false;
+    if (I == 0) is = nx;
+    else if (kind != 0 && I == nx + 1) is = 1;
+    if (J == 0) js = ny;
+    else if (kind != 1 && J == ny + 1) js = 1;
+    if (kind == 2) {
+        if (K == 0 || K == nz) zero = true;
+    } else if (kind == 3) {
+        if (K == 0) ks = 1;
+        else if (K == nz + 1) ks = nz;
+    } else {
+        if (K == 0) { ks = 1; sg = -1.0; }
+        else if (K == nz + 1) { ks = nz; sg = g.top_neumann ? 1.0 : -1.0; }
+    }
+    double v = zero ? 0.0 : F[((long)is * s.NJ + js) * s.NK + ks];
+    *out_val = (sg < 0.0) ? -v : v;
+}
+
+// Ghost entry t (0 <= t < ghost_count) of field F of the given kind.
+__device__ inline void bc_body(const Grid& g, double* F, int kind, long t) {
+    const GhostShape s = ghost_shape(kind, g.nx, g.ny, g.nz);
+    int I, J, K;
+    if (t < s.S1) {
+        long pl = (long)s.NJ * s.NK;
+        int p = (int)(t / pl);
+        long r = t % pl;
+        I = p == 0 ? s.xg0 : s.xg1;
+        J = (int)(r / s.NK);
+        K = (int)(r % s.NK);
+    } else if (t < s.S1 + s.S2) {
+        long t2 = t - s.S1;
+        long row = t2 / s.NK;
+        K = (int)(t2 % s.NK);
+        I = 1 + (int)(row / s.nyg);
+        J = (row % s.nyg) == 0 ? s.yg0 : s.yg1;
+    } else if (t < s.S1 + s.S2 + s.S3) {
+        long t3 = t - s.S1 - s.S2;
+        long col = t3 >> 1;
+        I = 1 + (int)(col / g.ny);
+        J = 1 + (int)(col % g.ny);
+        K = (t3 & 1) ? s.NK - 1 : 0;
+    } else {
+        return;
+    }
+    double v;
+    ghost_value(kind, g, F, s, I, J, K, &v);
+    F[((long)I * s.NJ + J) * s.NK + K] = v;
+}
+
+// ---------------------------------------------------------------------------
+// implicit z-diffusion: (I - theta dt nu Dzz) x = d + (1-theta) dt nu Dzz d
+// ---------------------------------------------------------------------------
+// The matrix is the same for every (i, j) column, so it is factored once
+// (tab = [a | c' | 1/den], n entries each) and every column only runs
+// forward/back substitution.  Columns are staged through LDS so the HBM
+// traffic is one coalesced read and one coalesced write per unknown.
+
+constexpr int kZColsMax = 16;
+constexpr int kZLdsBytes = 48 * 1024;  // dynamic LDS budget per workgroup
+
+__host__ __device__ inline int zsolve_ld(int comp, int nz) {
+    const int SZ = comp == 2 ? nz + 1 : nz + 2;
+    return SZ + ((SZ & 1) ? 0 : 1);  // odd stride: conflict-free column walk
+}
+
+// columns per workgroup (0: a column does not fit the LDS budget)
+__host__ __device__ inline int zsolve_cols(int comp, int nz) {
+    int cols = kZLdsBytes / (zsolve_ld(comp, nz) * (int)sizeof(double));
+    return cols > kZColsMax ? kZColsMax : cols;
+}
+
+// Column block cb (kZCols columns) of component C.  lds: kZCols * ld doubles;
+// cbase: kZColsMax longs.  Ends with a barrier-free write-back; the caller
+// synchronises before reusing lds.
+template <int C>
+__device__ inline void zsolve_body(const Grid& g, double* __restrict__ F,
+                                   const double* __restrict__ tab, double beta,
+                                   const double* __restrict__ dz_c,
+                                   const double* __restrict__ dz_f, int kZCols, long cb,
+                                   double* __restrict__ lds, long* cbase) {
+    const int SZ = (int)Comp<C>::SZ(g);
+    const long SY = Comp<C>::SY(g);
+    const int ld = zsolve_ld(C, g.nz);
+    const int n = (C == 2) ? g.nz - 1 : g.nz;
+    const long ncols = (long)g.nx * g.ny;
+    const long col0 = cb * kZCols;
+    const int tid = threadIdx.x;
+
+    // column base addresses (one 32-bit division per column, not per element)
+    if (tid < kZCols) {
+        const long gc = col0 + tid;
+        long a = -1;
+        if (gc < ncols) {
+            const unsigned ug = (unsigned)gc;
+            const int I = 1 + (int)(ug / (unsigned)g.ny), J = 1 + (int)(ug % (unsigned)g.ny);
+            a = ((long)I * SY + J) * SZ;
+        }
+        cbase[tid] = a;
+    }
+    __syncthreads();
+    // coalesced load of kZCols whole columns (ghost ends included)
+    for (int e = tid; e < kZCols * SZ; e += blockDim.x) {
+        const int c = e / SZ, k = e - (e / SZ) * SZ;
+        const long a = cbase[c];
+        if (a >= 0) lds[c * ld + k] = F[a + k];
+    }
+    __syncthreads();
+
+    if (tid < kZCols && col0 + tid < ncols) {
+        double* s = lds + tid * ld;
+        if (beta != 0.0) {
+            // explicit part, operators.py:446-466 (u, v) and 609-623 (w)
+            if (C < 2) {
+                double lo = s[0], mid = s[1];
+                for (int k = 0; k < n; ++k) {
+                    double hi = s[k + 2];
+                    double dzl = dz_c[k], dzr = dz_c[k + 1], dzc = dz_f[k];
+                    double d2;
+                    if (k == 0) {
+                        d2 = ((hi - mid) / dzr - 2.0 * mid / dzl) / dzc;
+                    } else if (k == n - 1) {
+                        if (g.top_neumann) d2 = (-(mid - lo) / dzl) / dzc;
+                        else d2 = (-2.0 * mid / dzr - (mid - lo) / dzl) / dzc;
+                    } else {
+                        d2 = ((hi - mid) / dzr - (mid - lo) / dzl) / dzc;
+                    }
+                    s[k + 1] = mid + beta * d2;
+                    lo = mid;
+                    mid = hi;
+                }
+            } else {
+                // the reference's w stencil is centred on face m (not m+1)
+                double a = 0.0, b = s[0], c = s[1];
+                for (int m = 0; m < n; ++m) {
+                    double dzl = dz_f[m], dzr = dz_f[m + 1], dzc = dz_c[m + 1];
+                    double d2;
+                    if (m == 0) d2 = ((c - b) / dzr - b / dzl) / dzc;
+                    else if (m == n - 1) d2 = (-b / dzr - (b - a) / dzl) / dzc;
+                    else d2 = ((c - b) / dzr - (b - a) / dzl) / dzc;
+                    double nxt = s[m + 2];
+                    s[m + 1] = c + beta * d2;
+                    a = b;
+                    b = c;
+                    c = nxt;
+                }
+            }
+        }
+        const double* A = tab;
+        const double* CP = tab + n;
+        const double* INV = tab + 2 * n;
+        double dp = s[1] * INV[0];
+        s[1] = dp;
+        for (int k = 1; k < n; ++k) {
+            dp = (s[k + 1] - A[k] * dp) * INV[k];
+            s[k + 1] = dp;
+        }
+        double x = dp;
+        for (int k = n - 2; k >= 0; --k) {
+            x = s[k + 1] - CP[k] * x;
+            s[k + 1] = x;
+        }
+    }
+    __syncthreads();
+
+    for (int e = tid; e < kZCols * SZ; e += blockDim.x) {
+        const int c = e / SZ, k = e - (e / SZ) * SZ;
+        const long a = cbase[c];
+        if (a >= 0 && k >= 1 && k <= n) F[a + k] = lds[c * ld + k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// divergence -> plane-major (k, i, j) Poisson right-hand side
+// ---------------------------------------------------------------------------
+
+constexpr int kT = 32;  // transpose tile edge
+
+// tiles of the (i; j, k) transposes: nx * ceil(ny/kT) * ceil(nz/kT)
+__host__ __device__ inline long plane_tiles(int nx, int ny, int nz) {
+    return (long)nx * ((ny + kT - 1) / kT) * ((nz + kT - 1) / kT);
+}
+
+// tile index -> (i, j0, k0), k-tiles fastest
+__device__ inline void plane_tile(const Grid& g, long t, int& i, int& j0, int& k0) {
+    const int ntk = (g.nz + kT - 1) / kT, ntj = (g.ny + kT - 1) / kT;
+    k0 = (int)(t % ntk) * kT;
+    const long r = t / ntk;
+    j0 = (int)(r % ntj) * kT;
+    i = (int)(r / ntj);
+}
+
+__device__ inline double divergence_at(const Grid& g, const double* __restrict__ u,
+                                       const double* __restrict__ v,
+                                       const double* __restrict__ w,
+                                       const double* __restrict__ dz_f, int I, int J, int K) {
+    const long uSY = g.ny + 2, uSZ = g.nz + 2;
+    const long vSY = g.ny + 1, vSZ = g.nz + 2;
+    const long wSY = g.ny + 2, wSZ = g.nz + 1;
+    double du = (u[((long)I * uSY + J) * uSZ + K] - u[((long)(I - 1) * uSY + J) * uSZ + K]) / g.dx;
+    double dv = (v[((long)I * vSY + J) * vSZ + K] - v[((long)I * vSY + (J - 1)) * vSZ + K]) / g.dy;
+    long wb = ((long)I * wSY + J) * wSZ;
+    double dw = (w[wb + K] - w[wb + K - 1]) / dz_f[K - 1];
+    return du + dv + dw;
+}
+
+// tile: kT x (kT+1) doubles of LDS
+__device__ inline void div_planes_body(const Grid& g, const double* __restrict__ u,
+                                       const double* __restrict__ v,
+                                       const double* __restrict__ w,
+                                       const double* __restrict__ dz_f, double scale,
+                                       double* __restrict__ planes, int i, int j0, int k0,
+                                       double (*tile)[kT + 1]) {
+    const int t = threadIdx.x;
+    {
+        const int kk = t % kT;
+        for (int jj = t / kT; jj < kT; jj += 256 / kT) {
+            int j = j0 + jj, k = k0 + kk;
+            if (j < g.ny && k < g.nz) {
+                double d = divergence_at(g, u, v, w, dz_f, i + 1, j + 1, k + 1);
+                tile[jj][kk] = d / scale;
+            }
+        }
+    }
+    __syncthreads();
+    {
+        const int jj = t % kT;
+        for (int kk = t / kT; kk < kT; kk += 256 / kT) {
+            int j = j0 + jj, k = k0 + kk;
+            if (j < g.ny && k < g.nz) planes[((long)k * g.nx + i) * g.ny + j] = tile[jj][kk];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// per-wavenumber Thomas (projection.py:199-247 with the matrices of 47-114)
+// ---------------------------------------------------------------------------
+// dhat: (nz, nx, nky) complex, plane-major; one lane per mode walks k, so a
+// wavefront reads 64 consecutive modes (1 KiB) per plane.  c' is kept in a
+// real scratch of the same (nz, modes) shape.
+
+__device__ inline void poisson_thomas_body(const PoissonTabs& t, double2* __restrict__ d,
+                                           double* __restrict__ cpw, long m) {
+    const long M = (long)t.nx * t.nky;
+    if (m >= M) return;
+    const int nz = t.nz;
+    const int i = (int)(m / t.nky), jc = (int)(m % t.nky);
+    const double khsq = t.kxsq[i] + t.kysq[jc];
+    // row 0 (Neumann bottom; identity row for the (0,0) mode)
+    double b = -(t.cs[0] + khsq);
+    b += t.cl[0];
+    if (nz == 1) b += t.cr[0];
+    double c = nz > 1 ? t.cr[0] : 0.0;
+    if (m == 0) {
+        b = 1.0;
+        c = 0.0;
+    }
+    double cp = c / b;
+    double2 dk = d[m];
+    double dr = dk.x / b, di = dk.y / b;
+    d[m] = make_double2(dr, di);
+    cpw[m] = cp;
+    for (int k = 1; k < nz; ++k) {
+        const double a = t.cl[k];
+        double bk = -(t.cs[k] + khsq);
+        double ck = 0.0;
+        if (k == nz - 1) bk += t.cr[nz - 1];
+        else ck = t.cr[k];
+        double den = bk - a * cp;
+        if (k < nz - 1) cp = ck / den;
+        long idx = (long)k * M + m;
+        dk = d[idx];
+        dr = (dk.x - a * dr) / den;
+        di = (dk.y - a * di) / den;
+        d[idx] = make_double2(dr, di);
+        cpw[idx] = cp;
+    }
+    double xr = dr, xi = di;
+    for (int k = nz - 2; k >= 0; --k) {
+        long idx = (long)k * M + m;
+        double cpk = cpw[idx];
+        dk = d[idx];
+        xr = dk.x - cpk * xr;
+        xi = dk.y - cpk * xi;
+        d[idx] = make_double2(xr, xi);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// projection u -= dt grad p from the plane-major (unnormalised) inverse FFT
+// ---------------------------------------------------------------------------
+// Also writes the interior of the ghost-shaped pressure p = P / (nx ny).
+// t0: (kT+1) x (kT+2), t1: kT x (kT+1) doubles of LDS.
+
+__device__ inline void project_planes_body(const Grid& g, double* __restrict__ u,
+                                           double* __restrict__ v, double* __restrict__ w,
+                                           const double* __restrict__ P, double inv_n, double dt,
+                                           const double* __restrict__ dz_c,
+                                           double* __restrict__ p, int i, int j0, int k0,
+                                           double (*t0)[kT + 2], double (*t1)[kT + 1]) {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int i1 = (i + 1 == nx) ? 0 : i + 1;
+    const int t = threadIdx.x;
+    // load (j fastest: plane rows are contiguous in j)
+    for (int e = t; e < (kT + 1) * (kT + 1); e += 256) {
+        int jj = e % (kT + 1), kk = e / (kT + 1);
+        int j = j0 + jj, k = k0 + kk;
+        if (k < nz && j <= ny && jj <= kT) {
+            int jw = (j == ny) ? 0 : j;
+            t0[jj][kk] = P[((long)k * nx + i) * ny + jw];
+        }
+    }
+    for (int e = t; e < kT * kT; e += 256) {
+        int jj = e % kT, kk = e / kT;
+        int j = j0 + jj, k = k0 + kk;
+        if (k < nz && j < ny) t1[jj][kk] = P[((long)k * nx + i1) * ny + j];
+    }
+    __syncthreads();
+    // update (k fastest: fields are z-contiguous)
+    const int I = i + 1;
+    const int kk = t % kT;
+    for (int jj = t / kT; jj < kT; jj += 256 / kT) {
+        int j = j0 + jj, k = k0 + kk;
+        if (j >= ny || k >= nz) continue;
+        const int J = j + 1, K = k + 1;
+        double pc = t0[jj][kk] * inv_n;
+        double pxp = t1[jj][kk] * inv_n;
+        double pyp = t0[jj + 1][kk] * inv_n;
+        if (u) {
+            long ui = ((long)I * (ny + 2) + J) * (nz + 2) + K;
+            u[ui] = u[ui] - dt * ((pxp - pc) / g.dx);
+            long vi = ((long)I * (ny + 1) + J) * (nz + 2) + K;
+            v[vi] = v[vi] - dt * ((pyp - pc) / g.dy);
+        }
+        if (u && k < nz - 1) {
+            double pzp = t0[jj][kk + 1] * inv_n;
+            long wi = ((long)I * (ny + 2) + J) * (nz + 1) + K;
+            w[wi] = w[wi] - dt * ((pzp - pc) / dz_c[K]);
+        }
+        if (p) p[((long)I * (ny + 2) + J) * (nz + 2) + K] = pc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// bulk velocity and the exact flux shift
+// ---------------------------------------------------------------------------
+// sum_{ijk} u_ijk * (dx dy dz_f[k]) / V, reduced over a fixed partition and a
+// fixed tree (deterministic).  scal: [0] correction, [1] forcing, [2] u_bulk
+// after the shift, [3] u_bulk before it.
+
+constexpr int kRedBlock = 256;
+
+__host__ __device__ inline int bulk_blocks(int nx, int ny, int nz) {
+    long N = (long)nx * ny * nz;
+    long nb = (N + kRedBlock * 8 - 1) / (kRedBlock * 8);
+    return (int)(nb < 2048 ? nb : 2048);
+}
+
+// red: kRedBlock doubles of LDS; every thread gets the block sum
+__device__ inline double block_sum(double v, double* red) {
+    __syncthreads();
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kRedBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+// Partial sum number blk of nb over u*cv; with shift, u += corr first.
+__device__ inline void bulk_partial_body(const Grid& g, double* __restrict__ u,
+                                         const double* __restrict__ cv, double corr, int shift,
+                                         double* __restrict__ partials, long blk, long nb,
+                                         double* red) {
+    const long N = (long)g.nx * g.ny * g.nz;
+    const long chunk = (N + nb - 1) / nb;
+    const long beg = blk * chunk;
+    const long end = beg + chunk < N ? beg + chunk : N;
+    double acc = 0.0;
+    for (long n = beg + threadIdx.x; n < end; n += kRedBlock) {
+        const unsigned un = (unsigned)n;
+        int k = (int)(un % (unsigned)g.nz);
+        unsigned r = un / (unsigned)g.nz;
+        int j = (int)(r % (unsigned)g.ny), i = (int)(r / (unsigned)g.ny);
+        long ui = ((long)(i + 1) * (g.ny + 2) + (j + 1)) * (g.nz + 2) + (k + 1);
+        double val = u[ui];
+        if (shift) {
+            val = val + corr;
+            u[ui] = val;
+        }
+        acc += val * cv[k];
+    }
+    double s = block_sum(acc, red);
+    if (threadIdx.x == 0) partials[blk] = s;
+}
+
+// mode 0: scal[3] = ub*, scal[0] = U_b - ub*, scal[1] = scal[0] / dt
+// mode 1: scal[2] = ub
+__device__ inline void bulk_finalize_body(const double* __restrict__ partials, int np,
+                                          double total_volume, double U_bulk, double dt,
+                                          int mode, double* __restrict__ scal, double* red) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < np; b += kRedBlock) acc += partials[b];
+    double s = block_sum(acc, red);
+    if (threadIdx.x == 0) {
+        double ub = s / total_volume;
+        if (mode == 0) {
+            double corr = U_bulk - ub;
+            scal[3] = ub;
+            scal[0] = corr;
+            scal[1] = corr / dt;
+        } else {
+            scal[2] = ub;
+        }
+    }
+}
+
+}  // namespace slc

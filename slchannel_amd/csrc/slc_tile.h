@@ -1,0 +1,316 @@
+// slc_tile.h -- the LDS-tiled "trace, then gather" tile of the semi-Lagrangian
+// step, shared by the single-run kernels (slc_step.hip) and the batched
+// ensemble kernels (slc_ens_kernels.hip).
+//
+// One kernel template covers the four uses of "trace, then gather":
+//   MODE_ADVECT  foot at dt of the given trajectory velocity, gather F
+//                (SLAdvector.advect, semilag.py:584-671)
+//   MODE_BDF1    as ADVECT with U* = V^n, plus dt * nu L_xy V^n
+//                (solver.py:737-765)
+//   MODE_FAR     BOTH feet of the frozen U* (depths dt and 2dt, each an
+//                independent iterated midpoint from the arrival point; their
+//                first sample, at the arrival point itself, is shared), the
+//                near foot stored for MODE_NEAR, V^{n-1} gathered at the far
+//                foot into out                          (solver.py:870-882)
+//   MODE_NEAR    V^n gathered at the stored near foot, then
+//                out = 4/3 near - 1/3 out + (2dt/3)(2 R^n - R^{n-1})
+//                                                       (solver.py:885-903)
+// A workgroup owns a TI x TJ x TK tile of arrival points (z fastest).  Phase 1
+// traces the tile's feet and keeps each foot's stencil (base indices,
+// fractional offsets, clamped z) in registers; the tile's stencil bounding
+// box (its trajectory-CFL halo, measured) is reduced and staged in LDS with
+// coalesced z-contiguous loads; phase 2 contracts every P^3 stencil from LDS
+// (z, then y, then x, with FMAs -- the same order as the direct gather).  A
+// box larger than the LDS budget falls back to global loads for that tile.
+#pragma once
+
+#include <climits>
+
+#include "slc_device.h"
+#include "slc_internal.h"
+
+namespace slc {
+
+constexpr int kSB = 256;  // threads per workgroup
+
+enum { MODE_ADVECT = 0, MODE_BDF1 = 1, MODE_FAR = 2, MODE_NEAR = 3, MODE_GATHER = 4 };
+
+// Foot coordinates of one component (flat arrival order): written by
+// MODE_FAR (near feet), read by MODE_NEAR and MODE_GATHER.
+struct Feet {
+    double* x;
+    double* y;
+    double* z;
+};
+
+struct SBox {
+    int xmin, xmax, ymin, ymax, kmin, kmax;
+};
+
+__device__ inline int s_wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        int w = __shfl_xor(v, o);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+__device__ inline int s_wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        int w = __shfl_xor(v, o);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+__device__ inline SBox s_block_box(SBox l, int* sh) {
+    if (threadIdx.x == 0) {
+        sh[0] = INT_MAX; sh[1] = INT_MIN;
+        sh[2] = INT_MAX; sh[3] = INT_MIN;
+        sh[4] = INT_MAX; sh[5] = INT_MIN;
+    }
+    __syncthreads();
+    int a = s_wave_min(l.xmin), b = s_wave_max(l.xmax);
+    int c = s_wave_min(l.ymin), d = s_wave_max(l.ymax);
+    int e = s_wave_min(l.kmin), f = s_wave_max(l.kmax);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&sh[0], a); atomicMax(&sh[1], b);
+        atomicMin(&sh[2], c); atomicMax(&sh[3], d);
+        atomicMin(&sh[4], e); atomicMax(&sh[5], f);
+    }
+    __syncthreads();
+    SBox r{sh[0], sh[1], sh[2], sh[3], sh[4], sh[5]};
+    __syncthreads();
+    return r;
+}
+
+// Trajectory samples of the three U* components at one point (they share
+// x/dx, y/dy and the face cell of z; semilag.py:512-528 recomputes them with
+// identical results).
+__device__ inline void sample3(const Grid& g, const TrajVel& V, double x, double y, double z,
+                               double& us, double& vs, double& ws) {
+    const double zc = clampd(z, g.z_lo, g.z_hi);
+    const double xdx = x / g.dx, ydy = y / g.dy;
+    const int cell = face_cell_fast(g, zc);
+    us = trilinear<0>(g, V, xdx, ydy, zc, cell);
+    vs = trilinear<1>(g, V, xdx, ydy, zc, cell);
+    ws = trilinear<2>(g, V, xdx, ydy, zc, cell);
+}
+
+// Arrival-point trajectory sample with precomputed x/dx, y/dy and face cell
+// (the same values sample3 computes; see ArrivalTabs).
+__device__ inline void sample3_at(const Grid& g, const TrajVel& V, double xdx, double ydy,
+                                  double z, int cell, double& us, double& vs, double& ws) {
+    const double zc = clampd(z, g.z_lo, g.z_hi);
+    us = trilinear<0>(g, V, xdx, ydy, zc, cell);
+    vs = trilinear<1>(g, V, xdx, ydy, zc, cell);
+    ws = trilinear<2>(g, V, xdx, ydy, zc, cell);
+}
+
+// Per-foot stencil data kept between the phases.
+struct FootSt {
+    int bx, by, bk;      // unwrapped x/y base, clamped z base
+    double tx, ty, z;    // fractional offsets, clamped z
+};
+
+template <int C, int P>
+__device__ inline void make_st(const Grid& g, double x, double y, double z, FootSt& f,
+                               int& ncl) {
+    ncl += (z < g.z_lo || z > g.z_hi) ? 1 : 0;
+    f.z = clampd(z, g.z_lo, g.z_hi);
+    stencil_base<C, P>(g, x, y, f.z, f.bx, f.by, f.bk, f.tx, f.ty);
+}
+
+// T: storage type of the gathered field (double, or float for the FP32-storage
+// variant: values are staged and read as float, weights and accumulation stay
+// FP64).  F / Fm are always the FP64 fields (explicit xy-diffusion);
+// Sn / Sm are the gathered copies (== F / Fm when T is double).
+// One tile of the step kernel (tile = linear tile index, k fastest).  The
+// caller provides the block's LDS: lds (LDS_BYTES), sh (6 ints), s_cnt.
+template <typename T, int C, int P, int MODE, int TI, int TJ, int TK, int LDS_BYTES>
+__device__ __forceinline__ void sl_tile_body(
+    const Grid& g, const ArrivalTabs& A, const TrajVel& V, const double* __restrict__ F,
+    const double* __restrict__ Fm, const T* __restrict__ Sn, const T* __restrict__ Sm,
+    double dt, double* __restrict__ out, const Feet& ft, unsigned long long* nclamp, long tile,
+    T* __restrict__ lds, int* sh, int* s_cnt) {
+    constexpr int NPT = TI * TJ * TK;
+    static_assert(NPT % kSB == 0, "tile must be a multiple of the block");
+    constexpr int PT = NPT / kSB;
+    constexpr int CAP = LDS_BYTES / (int)sizeof(T);
+    const int nzi = Comp<C>::NZI(g);
+    const int ntk = (nzi + TK - 1) / TK, ntj = (g.ny + TJ - 1) / TJ;
+    const int tk = (int)(tile % ntk);
+    const long r0 = tile / ntk;
+    const int tj = (int)(r0 % ntj), ti = (int)(r0 / ntj);
+    const int i0 = ti * TI, j0 = tj * TJ, k0t = tk * TK;
+
+    // ---- phase 1: feet and stencils --------------------------------------
+    long idx[PT];
+    FootSt st[PT];
+    SBox l{INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN};
+    int ncl = 0;
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+        const int q = threadIdx.x + kSB * p;
+        const int kk = q % TK, t2 = q / TK;
+        const int jj = t2 % TJ, ii = t2 / TJ;
+        const int i = i0 + ii, j = j0 + jj, k = k0t + kk;
+        idx[p] = -1;
+        if (i < g.nx && j < g.ny && k < nzi) {
+            const long n = ((long)i * g.ny + j) * nzi + k;
+            idx[p] = n;
+            double xa, ya, za;
+            int I, J, K;
+            arrival<C>(g, n, xa, ya, za, I, J, K);
+            double x, y, z;
+            if (MODE == MODE_NEAR || MODE == MODE_GATHER) {
+                x = ft.x[n];
+                y = ft.y[n];
+                z = ft.z[n];
+            } else {
+                // iterated midpoint (semilag.py:534-552); the first sample is
+                // at the arrival point for every foot depth
+                double u0, v0, w0;
+                sample3_at(g, V, A.xdx[I - 1], A.ydy[J - 1], za, A.cell[K - 1], u0, v0, w0);
+                const double hf = (MODE == MODE_FAR) ? 0.5 * (2.0 * dt) : 0.5 * dt;
+                double xm = xa - hf * u0, ym = ya - hf * v0;
+                double zm = clampd(za - hf * w0, g.z_lo, g.z_hi);
+                for (int it = 1; it < g.n_iters; ++it) {
+                    double us, vs, ws;
+                    sample3(g, V, xm, ym, zm, us, vs, ws);
+                    xm = xa - hf * us;
+                    ym = ya - hf * vs;
+                    zm = clampd(za - hf * ws, g.z_lo, g.z_hi);
+                }
+                if (g.n_iters == 0) { xm = xa; ym = ya; zm = za; }
+                x = 2.0 * xm - xa;
+                y = 2.0 * ym - ya;
+                z = 2.0 * zm - za;
+                if (MODE == MODE_FAR) {
+                    const double hn = 0.5 * dt;
+                    double xn = xa - hn * u0, yn = ya - hn * v0;
+                    double zn = clampd(za - hn * w0, g.z_lo, g.z_hi);
+                    for (int it = 1; it < g.n_iters; ++it) {
+                        double us, vs, ws;
+                        sample3(g, V, xn, yn, zn, us, vs, ws);
+                        xn = xa - hn * us;
+                        yn = ya - hn * vs;
+                        zn = clampd(za - hn * ws, g.z_lo, g.z_hi);
+                    }
+                    if (g.n_iters == 0) { xn = xa; yn = ya; zn = za; }
+                    ft.x[n] = 2.0 * xn - xa;
+                    ft.y[n] = 2.0 * yn - ya;
+                    ft.z[n] = 2.0 * zn - za;
+                }
+            }
+            make_st<C, P>(g, x, y, z, st[p], ncl);
+            l.xmin = min(l.xmin, st[p].bx); l.xmax = max(l.xmax, st[p].bx);
+            l.ymin = min(l.ymin, st[p].by); l.ymax = max(l.ymax, st[p].by);
+            l.kmin = min(l.kmin, st[p].bk); l.kmax = max(l.kmax, st[p].bk);
+        }
+    }
+
+    // ---- stage the tile's stencil box in LDS -------------------------------
+    const T* __restrict__ S = (MODE == MODE_FAR) ? Sm : Sn;
+    const SBox b = s_block_box(l, sh);
+    const int BX = b.xmax - b.xmin + P, BY = b.ymax - b.ymin + P, BZ = b.kmax - b.kmin + P;
+    const bool fits = b.xmin <= b.xmax && (long)BX * BY * BZ <= CAP;
+    if (fits) {
+        // flat element loop (z fastest: coalesced rows); the quotients e/BZ
+        // and row/BY come from float reciprocals -- exact here because
+        // e < CAP < 2^13 keeps every fractional part >= 0.5/BZ from an integer
+        const int rows = BX * BY;
+        const float rbz = 1.0f / (float)BZ, rby = 1.0f / (float)BY;
+        for (int e = threadIdx.x; e < rows * BZ; e += kSB) {
+            int row = (int)(((float)e + 0.5f) * rbz);
+            row += (row + 1) * BZ <= e ? 1 : 0;  // exact even if the reciprocal
+            row -= row * BZ > e ? 1 : 0;         // were off by an ulp
+            const int zz = e - row * BZ;
+            int x = (int)(((float)row + 0.5f) * rby);
+            x += (x + 1) * BY <= row ? 1 : 0;
+            x -= x * BY > row ? 1 : 0;
+            const int y = row - x * BY;
+            const int gx = wrap_idx(b.xmin + x, g.nx), gy = wrap_idx(b.ymin + y, g.ny);
+            lds[e] = S[node_addr<C>(g, gx, gy, b.kmin + zz)];
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 2: contract -------------------------------------------------
+    const double* zn = znodes<C>(g);
+    const double* dinv = Comp<C>::centers ? g.dinv_cP : g.dinv_fP;
+    const double dx2 = g.dx * g.dx, dy2 = g.dy * g.dy;
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+        if (idx[p] < 0) continue;
+        const long n = idx[p];
+        const FootSt& f = st[p];
+        double wx[P], wy[P], wz[P];
+        uniform_weights<P>(f.tx, wx);
+        uniform_weights<P>(f.ty, wy);
+        z_weights<P>(zn, dinv, f.z, f.bk, wz);
+        double acc = 0.0;
+        if (fits) {
+            const int ox = f.bx - b.xmin, oy = f.by - b.ymin, oz = f.bk - b.kmin;
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                double ai = 0.0;
+#pragma unroll
+                for (int j = 0; j < P; ++j) {
+                    const T* col = lds + ((ox + i) * BY + (oy + j)) * BZ + oz;
+                    double sj = 0.0;
+#pragma unroll
+                    for (int k = 0; k < P; ++k) sj = fma(wz[k], (double)col[k], sj);
+                    ai = fma(wy[j], sj, ai);
+                }
+                acc = fma(wx[i], ai, acc);
+            }
+        } else {
+#pragma unroll 1
+            for (int i = 0; i < P; ++i) {
+                const int gx = wrap_idx(f.bx + i, g.nx);
+                double ai = 0.0;
+#pragma unroll
+                for (int j = 0; j < P; ++j) {
+                    const int gy = wrap_idx(f.by + j, g.ny);
+                    const T* col = S + node_addr<C>(g, gx, gy, f.bk);
+                    double sj = 0.0;
+#pragma unroll
+                    for (int k = 0; k < P; ++k) sj = fma(wz[k], (double)col[k], sj);
+                    ai = fma(wy[j], sj, ai);
+                }
+                acc = fma(uniform_weight_m<P>(f.tx, i), ai, acc);
+            }
+        }
+        // arrival ghost index of n
+        const unsigned un = (unsigned)n;
+        const int K = (int)(un % (unsigned)nzi) + 1;
+        const unsigned rr = un / (unsigned)nzi;
+        const int J = (int)(rr % (unsigned)g.ny) + 1, I = (int)(rr / (unsigned)g.ny) + 1;
+        const long o = ((long)I * Comp<C>::SY(g) + J) * Comp<C>::SZ(g) + K;
+        if (MODE == MODE_GATHER) {
+            out[n] = acc;
+        } else if (MODE == MODE_ADVECT || MODE == MODE_FAR) {
+            out[o] = acc;
+        } else if (MODE == MODE_BDF1) {
+            out[o] = acc + dt * rhs_xy<C>(g, F, I, J, K, dx2, dy2);
+        } else {
+            const double far = out[o];
+            const double rn = rhs_xy<C>(g, F, I, J, K, dx2, dy2);
+            const double rm = rhs_xy<C>(g, Fm, I, J, K, dx2, dy2);
+            const double rh = 2.0 * rn - rm;
+            const double dt_eff = (2.0 / 3.0) * dt;
+            const double pred = fma(far, -1.0 / 3.0, acc * (4.0 / 3.0));
+            out[o] = pred + dt_eff * rh;
+        }
+    }
+    if (nclamp) {
+        if (threadIdx.x == 0) *s_cnt = 0;
+        __syncthreads();
+        if (ncl) atomicAdd(s_cnt, ncl);
+        __syncthreads();
+        if (threadIdx.x == 0 && *s_cnt) atomicAdd(nclamp, (unsigned long long)*s_cnt);
+    }
+}
+
+}  // namespace slc
