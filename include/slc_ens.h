@@ -1,0 +1,96 @@
+/* slc_ens.h -- batched ensemble / serving layer of libslc_hip.so.
+ *
+ * An ensemble holds many independent channel runs ("members"), each of its
+ * own grid size, Reynolds number and time step, resident in HBM, and
+ * advances all of them by one SL-BDF2 step per slc_ens_step call with ONE
+ * batched launch per substep (ghost fills, U*, far and near SL passes per
+ * component, z-solves, divergence, Poisson Thomas, projection, bulk shift)
+ * plus one batched rocFFT execution per distinct grid size.  Members can be
+ * added and removed between steps (continuous batching); a newly added
+ * member takes the BDF1 bootstrap on its first step while the others take
+ * BDF2 steps in the same launches.
+ *
+ * Each member's step is the single-run step of SLChannelFlow.step_sl_bdf2
+ * (solver.py:813-1018; the same device bodies as slc_bdf2_predict,
+ * slc_zsolve, slc_project_step and slc_bulk_force), so a member's result does
+ * not depend on the other members.
+ *
+ * The reference has no counterpart: it steps one run per process
+ * (solver.py:1280-1500 run loop).  This API serves callers that advance many
+ * runs at once: ensembles of uneven members and a serving loop with
+ * continuous batching.
+ *
+ * Ownership: the ensemble allocates and owns every member's fields; callers
+ * copy in (slc_ens_set_fields) and out (slc_ens_get_fields) with device
+ * pointers.  Stream use: every call orders its work after `stream` and makes
+ * `stream` wait for it; the ensemble runs on an internal stream of default
+ * priority.  Not thread-safe: one host thread per ensemble.
+ */
+#ifndef SLC_ENS_H
+#define SLC_ENS_H
+
+#include "slc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct slc_ens slc_ens;
+
+/* order: 4 or 6 for every member; storage_bits: 64, or 32 for FP32-storage
+ * gathers (SLC_OPT_FIELD_STORAGE). */
+int slc_ens_create(int device, int order, int storage_bits, slc_ens** out);
+int slc_ens_destroy(slc_ens* ens);
+
+/* Device bytes one member of this grid would hold (fields, histories,
+ * tables, its share of the spectral buffers), for admission by HBM
+ * footprint.  desc->z_f / z_c may be null here. */
+int slc_ens_member_bytes(const slc_grid_desc* desc, int storage_bits, int64_t* bytes);
+
+/* Adds a member with zero fields; its first step is a BDF1 bootstrap.
+ * desc->order must equal the ensemble's order; desc->nu = 1/Re. */
+int slc_ens_add(slc_ens* ens, const slc_grid_desc* desc, double dt, double U_bulk, int* id);
+int slc_ens_remove(slc_ens* ens, int id);
+int slc_ens_count(const slc_ens* ens, int* n);
+/* Total device bytes held by the ensemble. */
+int slc_ens_bytes(const slc_ens* ens, int64_t* bytes);
+
+/* Copies ghost-shaped u, v, w, p (device pointers, float64, SLChannelFlow
+ * shapes) into the member and resets it to a BDF1 bootstrap (a restart,
+ * solver.py:834-844).  p may be null (zero pressure). */
+int slc_ens_set_fields(slc_ens* ens, int id, const double* u, const double* v, const double* w,
+                       const double* p, void* stream);
+/* Copies the member's current u, v, w, p out (device pointers). */
+int slc_ens_get_fields(slc_ens* ens, int id, double* u, double* v, double* w, double* p,
+                       void* stream);
+/* Copies the member's statistics of its last step into out (device pointer,
+ * 5 doubles): u_bulk after the shift, forcing, clamped feet, steps done,
+ * u_bulk before the shift. */
+int slc_ens_stats(slc_ens* ens, int id, double* out, void* stream);
+
+/* One SL-BDF2 step of every member. */
+int slc_ens_step(slc_ens* ens, void* stream);
+
+/* Options:
+ * SLC_ENS_OPT_TILE       SL tile shape 0: 4x4x32, 1: 2x8x32, 2: 8x4x16
+ * SLC_ENS_OPT_GRID       persistent workgroups per CU of the non-SL kernels (1..32)
+ * SLC_ENS_OPT_GRAPH      1: replay each step from a captured hipGraph (re-captured
+ *                        after membership changes), 0: launch eagerly */
+#define SLC_ENS_OPT_TILE 1
+#define SLC_ENS_OPT_GRID 2
+#define SLC_ENS_OPT_GRAPH 3
+int slc_ens_set_option(slc_ens* ens, int option, int value);
+int slc_ens_get_option(const slc_ens* ens, int option, int* value);
+
+/* Autotuner: times the SL passes (far + near, three components) of the
+ * current members `reps` times for every tile shape and keeps the fastest.
+ * Leaves every member's state unchanged (the passes only write the
+ * predictor scratch and the feet buffer; clamp counters are cleared).
+ * report (optional): "shape ms\n" lines. */
+int slc_ens_autotune(slc_ens* ens, int reps, void* stream, char* report, int64_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SLC_ENS_H */

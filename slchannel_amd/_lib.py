@@ -1,4 +1,5 @@
-"""ctypes binding of libslc_hip.so (the C ABI declared in include/slc.h).
+"""ctypes binding of libslc_hip.so (the C ABI declared in include/slc.h and
+include/slc_ens.h).
 
 The library is loaded from the package directory only; if it is missing the
 import of any GPU entry point raises -- there is no CPU fallback.
@@ -10,13 +11,19 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libslc_hip.so")
 
-# Every symbol include/slc.h declares (checked by the CPU test suite).
+# Every symbol include/slc.h and include/slc_ens.h declare (checked by the
+# CPU test suite).
 EXPORTS = (
     "slc_version", "slc_last_error", "slc_plan_create", "slc_plan_destroy",
     "slc_plan_workspace_bytes", "slc_apply_bc", "slc_departure", "slc_gather",
-    "slc_stencil", "slc_locate_z", "slc_advect", "slc_bdf1_predict", "slc_bdf2_predict", "slc_zsolve", "slc_divergence",
-    "slc_poisson", "slc_project", "slc_project_step", "slc_bulk_velocity", "slc_bulk_force",
-    "slc_set_option", "slc_profile_enable", "slc_profile_report",
+    "slc_stencil", "slc_locate_z", "slc_advect", "slc_bdf1_predict", "slc_bdf2_predict",
+    "slc_zsolve", "slc_divergence", "slc_poisson", "slc_project", "slc_project_step",
+    "slc_bulk_velocity", "slc_bulk_force", "slc_set_option", "slc_profile_enable",
+    "slc_profile_report",
+    "slc_ens_create", "slc_ens_destroy", "slc_ens_member_bytes", "slc_ens_add",
+    "slc_ens_remove", "slc_ens_count", "slc_ens_bytes", "slc_ens_set_fields",
+    "slc_ens_get_fields", "slc_ens_stats", "slc_ens_step", "slc_ens_set_option",
+    "slc_ens_get_option", "slc_ens_autotune",
 )
 
 
@@ -74,6 +81,20 @@ def load():
         "slc_set_option": ([vp, i, i], i),
         "slc_profile_enable": ([vp, i], i),
         "slc_profile_report": ([vp, ctypes.c_char_p, ctypes.c_int64], i),
+        "slc_ens_create": ([i, i, i, ctypes.POINTER(vp)], i),
+        "slc_ens_destroy": ([vp], i),
+        "slc_ens_member_bytes": ([ctypes.POINTER(GridDesc), i, ctypes.POINTER(ctypes.c_int64)], i),
+        "slc_ens_add": ([vp, ctypes.POINTER(GridDesc), d, d, ctypes.POINTER(i)], i),
+        "slc_ens_remove": ([vp, i], i),
+        "slc_ens_count": ([vp, ctypes.POINTER(i)], i),
+        "slc_ens_bytes": ([vp, ctypes.POINTER(ctypes.c_int64)], i),
+        "slc_ens_set_fields": ([vp, i, vp, vp, vp, vp, vp], i),
+        "slc_ens_get_fields": ([vp, i, vp, vp, vp, vp, vp], i),
+        "slc_ens_stats": ([vp, i, vp, vp], i),
+        "slc_ens_step": ([vp, vp], i),
+        "slc_ens_set_option": ([vp, i, i], i),
+        "slc_ens_get_option": ([vp, i, ctypes.POINTER(i)], i),
+        "slc_ens_autotune": ([vp, i, vp, ctypes.c_char_p, ctypes.c_int64], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -38,6 +38,9 @@ __host__ __device__ inline int ens_next(int s) {
                        : ens_state(ens_c(s), ens_b(s), ens_a(s), 1);
 }
 
+// 256-wide blocks covering n entries
+__host__ __device__ inline unsigned ublocks(long n) { return (unsigned)((n + 255) / 256); }
+
 // per-member statistics published by the last kernel of every step
 enum EnsStat {
     ES_UBULK = 0,   // bulk velocity after the shift (== U_bulk to rounding)
@@ -137,5 +140,6 @@ hipError_t run_ens_project(const EnsLaunch& L, unsigned grid, hipStream_t s);
 hipError_t run_ens_bulk_partial(const EnsLaunch& L, int shift, unsigned grid, hipStream_t s);
 hipError_t run_ens_bulk_final(const EnsLaunch& L, int mode, unsigned grid, hipStream_t s);
 hipError_t run_ens_advance(const EnsLaunch& L, hipStream_t s);
+hipError_t run_ens_clear_ncl(const EnsLaunch& L, hipStream_t s);
 
 }  // namespace slc

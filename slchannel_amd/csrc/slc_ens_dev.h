@@ -50,7 +50,6 @@ __device__ inline void ens_for(const unsigned* __restrict__ start, int nmem, uns
         unsigned total
 #define ENS_ARGS(L) (L).mem, (L).start, (L).nmem, (L).total
 
-__host__ __device__ inline unsigned ublocks(long n) { return (unsigned)((n + 255) / 256); }
 
 
 }  // namespace slc

@@ -271,4 +271,16 @@ hipError_t run_ens_advance(const EnsLaunch& L, hipStream_t s) {
     return hipGetLastError();
 }
 
+__global__ void ens_clear_ncl_kernel(const EnsMember* __restrict__ mem, int nmem) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < nmem) *mem[m].ncl = 0ull;
+}
+
+hipError_t run_ens_clear_ncl(const EnsLaunch& L, hipStream_t s) {
+    if (L.nmem == 0) return hipSuccess;
+    hipLaunchKernelGGL(ens_clear_ncl_kernel, dim3((L.nmem + 63) / 64), dim3(64), 0, s, L.mem,
+                       L.nmem);
+    return hipGetLastError();
+}
+
 }  // namespace slc

@@ -1,6 +1,7 @@
 """CPU-only checks of the native boundary and the host logic (no GPU calls).
 
-* libslc_hip.so loads and exports every function include/slc.h declares;
+* libslc_hip.so loads and exports every function include/slc.h and
+  include/slc_ens.h declare;
 * the ctypes table in slchannel_amd/_lib.py binds exactly those symbols;
 * the drop-in classes reject bad arguments with the reference's errors
   (slchannel/semilag.py:204-219) before touching a device;
@@ -18,8 +19,11 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_symbols():
-    hdr = open(os.path.join(REPO, "include", "slc.h")).read()
-    return set(re.findall(r"^(?:int|const char\*)\s+(slc_\w+)\(", hdr, re.M))
+    out = set()
+    for h in ("slc.h", "slc_ens.h"):
+        hdr = open(os.path.join(REPO, "include", h)).read()
+        out |= set(re.findall(r"^(?:int|const char\*)\s+(slc_\w+)\(", hdr, re.M))
+    return out
 
 
 def test_library_exports_every_declared_symbol(check):
@@ -28,7 +32,7 @@ def test_library_exports_every_declared_symbol(check):
     lib = _lib.load()
     check("library loads", lib is not None, _lib.LIB_PATH)
     decl = declared_symbols()
-    check("header declares entry points", len(decl) >= 20, f"{len(decl)} declared")
+    check("headers declare entry points", len(decl) >= 35, f"{len(decl)} declared")
     missing = [s for s in decl if not hasattr(lib, s)]
     check("every declared symbol exported", not missing, f"missing={missing}")
     check("ctypes table == header", set(_lib.EXPORTS) == decl,
