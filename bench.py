@@ -50,10 +50,17 @@ def parse():
                     help="bits of the gathered field copies: 64 (default) or 32 "
                          "(FP32-storage variant; adds an FP64 twin run and reports the "
                          "per-field difference)")
-    ap.add_argument("--config", type=int, default=2, choices=(2, 3),
+    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5),
                     help="2: 256^3 channel (default, the headline workload); "
                          "3: 1024x1024x512 large box, CFL 4, plus the isolated gather "
-                         "with feet displaced uniformly in +-5 cells")
+                         "with feet displaced uniformly in +-5 cells; "
+                         "4: 48-member uneven ensemble, batched vs one run at a time; "
+                         "5: serving loop with Poisson arrivals (p50/p99 step latency)")
+    ap.add_argument("--members", type=int, default=48, help="config 4: ensemble size")
+    ap.add_argument("--requests", type=int, default=24, help="config 5: number of requests")
+    ap.add_argument("--rate", type=float, default=1.0, help="config 5: arrivals per second")
+    ap.add_argument("--hbm-budget-gb", type=float, default=48.0,
+                    help="config 5: HBM budget of the admission control")
     return ap.parse_args()
 
 
@@ -76,6 +83,10 @@ def main():
         del src, dst
     if args.config == 3:
         return large_box(args)
+    if args.config == 4:
+        return ensemble_bench(args)
+    if args.config == 5:
+        return serve_bench(args)
     n, nz = args.n, args.nz or args.n
     Re_b, Re_tau, dtp = 2792.8, 180.0, 0.2
     cfg = {
@@ -263,6 +274,169 @@ def large_box(args):
                        for k, (c, t) in sorted(prof.items())},
         "device_mem_GB": round(torch.cuda.max_memory_allocated() / 1e9, 1),
         "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+ENSEMBLE_GRIDS = ((64, 48, 64), (96, 64, 96), (128, 96, 128), (192, 128, 128))
+
+
+def _member_cfg(nx, ny, nz, Re, order, fp32=False):
+    return {"grid": {"nx": nx, "ny": ny, "nz": nz},
+            "domain": {"Lx": 4 * math.pi, "Ly": 2 * math.pi, "Lz": 2.0},
+            "flow": {"Re": Re, "U_bulk": 1.0, "gamma": 1.5},
+            "sl": {"interp_order": order, "interp_dtype": "fp32_accum64" if fp32 else "fp64"}}
+
+
+def ensemble_bench(args):
+    """48 members drawn from {64x48x64, 96x64x96, 128x96x128, 192x128x128},
+    Re ~ U[2800, 6000], seeds 4000 + m, order 6, dt for trajectory CFL 3 on
+    each member's initial state.  Times K steps (after W warm-up steps, the
+    first a BDF1 bootstrap) of (a) the plain per-run path, each member's
+    SLChannelFlow stepped alone, one member after the other, and (b) the
+    batched ensemble (autotuned tile shape), eager and hipGraph replay.  Also
+    the FP32-storage ensemble's per-field difference to the FP64 one after
+    W + K steps."""
+    import numpy as np
+    import torch
+
+    from slchannel_amd import SLChannelFlow
+    from slchannel_amd.diagnostics import cfl_dt
+    from slchannel_amd.ensemble import OPT_GRAPH, Ensemble
+
+    rng = np.random.Generator(np.random.PCG64(4000))
+    specs = []
+    for m in range(args.members):
+        nx, ny, nz = ENSEMBLE_GRIDS[int(rng.integers(len(ENSEMBLE_GRIDS)))]
+        specs.append((nx, ny, nz, float(rng.uniform(2800.0, 6000.0)), 4000 + m))
+    inits, flows, dts = [], [], []
+    for nx, ny, nz, Re, seed in specs:
+        f = SLChannelFlow(config=_member_cfg(nx, ny, nz, Re, args.order), device="cuda:0")
+        f.initialize(seed=seed, urms=0.1, kmax=8.0)
+        dts.append(cfl_dt(f, 3.0))
+        inits.append((f.u.clone(), f.v.clone(), f.w.clone(), f.p.clone()))
+        flows.append(f)
+    points = sum(nx * ny * nz for nx, ny, nz, _, _ in specs)
+    K, W = args.steps, args.warmup
+
+    # (a) plain per-run path, one member at a time
+    torch.cuda.synchronize()
+    plain_s = 0.0
+    for f, dt in zip(flows, dts):
+        for _ in range(W):
+            f.step_sl_bdf2(dt)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(K):
+            f.step_sl_bdf2(dt)
+        torch.cuda.synchronize()
+        plain_s += time.perf_counter() - t
+    plain = args.members * K / plain_s
+
+    # (b) batched ensemble
+    def build(storage):
+        ens = Ensemble(order=args.order, storage=storage)
+        ids = [ens.add(_member_cfg(nx, ny, nz, Re, args.order, storage == 32), dt, fields=ini)
+               for (nx, ny, nz, Re, _), dt, ini in zip(specs, dts, inits)]
+        return ens, ids
+
+    ens, ids = build(64)
+    for _ in range(W):
+        ens.step()
+    tune = ens.autotune(reps=2)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(K):
+        ens.step()
+    torch.cuda.synchronize()
+    eager_s = time.perf_counter() - t
+    ens.set_option(OPT_GRAPH, 1)
+    ens.step()  # capture
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(K):
+        ens.step()
+    torch.cuda.synchronize()
+    graph_s = time.perf_counter() - t
+    batched = args.members * max(K / eager_s, K / graph_s)
+    # bit-identity spot check against the plain path after W + K steps (member 0)
+    f0 = ens.get_fields(ids[0])  # member 0 is now W + 2K + 1 steps in; step the plain one on
+    for _ in range(K + 1):
+        flows[0].step_sl_bdf2(dts[0])
+    same0 = all(torch.equal(f0[k], getattr(flows[0], k)) for k in "uvwp")
+
+    # FP32-storage ensemble vs FP64 ensemble after W + K steps
+    e64, i64 = build(64)
+    e32, i32 = build(32)
+    for _ in range(W + K):
+        e64.step()
+        e32.step()
+    err = {k: 0.0 for k in "uvwp"}
+    for a, b in zip(i64, i32):
+        fa, fb = e64.get_fields(a), e32.get_fields(b)
+        for k in "uvwp":
+            err[k] = max(err[k], (fb[k] - fa[k]).abs().max().item() / fa[k].abs().max().item())
+    line = {
+        "metric": "member-steps/s (48-member uneven SL-BDF2 ensemble, triquintic fp64)",
+        "value": round(batched, 2), "unit": "member-steps/s", "n_gpus": 1, "steps": K,
+        "warmup": W, "ms_per_step": round(min(eager_s, graph_s) * 1e3 / K, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded solenoidal perturbations on Poiseuille, projected)",
+        "config": {"workload": f"{args.members} members from {list(ENSEMBLE_GRIDS)}, "
+                               "Re U[2800,6000], order 6, CFL 3",
+                   "members": args.members, "grid_points": points},
+        "plain_per_run_member_steps_per_s": round(plain, 2),
+        "batched_vs_plain": round(batched / plain, 3),
+        "eager_member_steps_per_s": round(args.members * K / eager_s, 2),
+        "graph_member_steps_per_s": round(args.members * K / graph_s, 2),
+        "grid_point_steps_per_s": round(points * K / min(eager_s, graph_s), 1),
+        "autotune_ms": tune,
+        "member0_identical_to_plain": bool(same0),
+        "fp32_storage_rel_err_vs_fp64": {k: float(f"{v:.3e}") for k, v in err.items()},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def serve_bench(args):
+    """Poisson arrivals (rate per second, seeded) of requests with grids from
+    {64^3, 128x96x128, 256^3, 384x384x256}, Re ~ U[2800, 10000], N ~ U[20, 200]
+    steps, order 6, dt for trajectory CFL 3; admission by HBM footprint under
+    the given budget.  The same requests are then run one at a time on the
+    plain per-run path."""
+    import numpy as np
+
+    from slchannel_amd.serve import Server, poisson_requests, run_alone
+
+    grids = ((64, 64, 64), (128, 96, 128), (256, 256, 256), (384, 384, 256))
+    reqs = poisson_requests(args.requests, args.rate, grids, (2800.0, 10000.0), (20, 200),
+                            seed=5000, order=args.order)
+    srv = Server(hbm_budget=int(args.hbm_budget_gb * 1e9))
+    results, m = srv.run(reqs)
+    plain_s, plain_lat = 0.0, []
+    for r in reqs:
+        res, el = run_alone(r, keep_fields=False)
+        plain_s += el
+        plain_lat += res.latencies_ms
+    total_steps = sum(r.n_steps for r in reqs)
+    plain = total_steps / plain_s
+    line = {
+        "metric": "member-steps/s served (continuous batching, Poisson arrivals)",
+        "value": round(m["member_steps_per_s"], 2), "unit": "member-steps/s", "n_gpus": 1,
+        "steps": total_steps, "warmup": 0, "ms_per_step": round(m["p50_step_ms"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded solenoidal perturbations on Poiseuille, projected)",
+        "config": {"workload": f"{args.requests} requests, rate {args.rate}/s, grids "
+                               f"{list(grids)}, Re U[2800,10000], N U[20,200]",
+                   "hbm_budget_GB": args.hbm_budget_gb},
+        "p50_step_ms": round(m["p50_step_ms"], 3), "p99_step_ms": round(m["p99_step_ms"], 3),
+        "busy_member_steps_per_s": round(m["member_steps_per_busy_s"], 2),
+        "wall_s": round(m["wall_s"], 2),
+        "plain_per_run_member_steps_per_s": round(plain, 2),
+        "plain_p50_step_ms": round(float(np.percentile(plain_lat, 50)), 3),
+        "plain_p99_step_ms": round(float(np.percentile(plain_lat, 99)), 3),
+        "served_vs_plain": round(m["member_steps_per_s"] / plain, 3),
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
 

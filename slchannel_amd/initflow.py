@@ -40,8 +40,11 @@ def _modes(kmin, kmax, nx, ny):
 
 
 def solenoidal_fields(nx, ny, nz, Lx, Ly, Lz, z_f, z_c, seed, urms=0.1, kmin=1.0, kmax=8.0,
-                      spectrum="flat", U_bulk=1.0, chunk=32):
-    """Return ghost-shaped numpy arrays (u, v, w) with interiors set."""
+                      spectrum="flat", U_bulk=1.0, chunk=32, device=None):
+    """Return ghost-shaped arrays (u, v, w) with interiors set: numpy arrays,
+    or float64 torch tensors on `device` when one is given (the x-y
+    synthesis then runs as torch FFTs there; same modes and amplitudes, FFT
+    rounding aside)."""
     z_f = np.asarray(z_f, dtype=np.float64)
     z_c = np.asarray(z_c, dtype=np.float64)
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -69,6 +72,10 @@ def solenoidal_fields(nx, ny, nz, Lx, Ly, Lz, z_f, z_c, seed, urms=0.1, kmin=1.0
         g = env[None, :] * s
         dg = denv[None, :] * s + env[None, :] * c * (np.outer(ll, np.ones_like(z)) * math.pi / Lz)
         return g, dg
+
+    if device is not None:
+        return _solenoidal_torch(nx, ny, nz, Lx, Lz, z_f, z_c, urms, U_bulk, chunk, device, mm,
+                                 nn, kx, ky, A, G, dx, dy)
 
     def evaluate(coef_fn, zs, xshift, yshift):
         # value at x_i = i dx + xshift (i = 0..nx-1), y_j = j dy + yshift, z in zs
@@ -106,6 +113,50 @@ def solenoidal_fields(nx, ny, nz, Lx, Ly, Lz, z_f, z_c, seed, urms=0.1, kmin=1.0
     w = np.zeros((nx + 2, ny + 2, nz + 1))
     zeta = 2.0 * zc_in / Lz - 1.0
     base = 1.5 * U_bulk * (1.0 - zeta**2)
+    u[1 : nx + 1, 1 : ny + 1, 1 : nz + 1] = scale * up + base[None, None, :]
+    v[1 : nx + 1, 1 : ny + 1, 1 : nz + 1] = scale * vp
+    w[1 : nx + 1, 1 : ny + 1, 1:nz] = scale * wp
+    return u, v, w
+
+
+def _solenoidal_torch(nx, ny, nz, Lx, Lz, z_f, z_c, urms, U_bulk, chunk, device, mm, nn, kx, ky, A,
+                      G, dx, dy):
+    import torch
+
+    dev = torch.device(device)
+    mi = torch.as_tensor(mm % nx, device=dev)
+    ni = torch.as_tensor(nn % ny, device=dev)
+
+    def evaluate(coef_fn, zs, xshift, yshift):
+        out = torch.empty((nx, ny, len(zs)), dtype=torch.float64, device=dev)
+        phase = np.exp(1j * (kx * xshift + ky * yshift))
+        for z0 in range(0, len(zs), chunk):
+            zz = zs[z0 : z0 + chunk]
+            g, dg = G(zz)
+            coef = torch.as_tensor(coef_fn(g, dg) * phase[:, None], device=dev)
+            C = torch.zeros((nx, ny, len(zz)), dtype=torch.complex128, device=dev)
+            C.index_put_((mi, ni), coef, accumulate=True)
+            out[:, :, z0 : z0 + len(zz)] = torch.fft.ifft2(C, dim=(0, 1)).real * (nx * ny)
+        return out
+
+    ikx, iky = 1j * kx, 1j * ky
+    fu = lambda g, dg: (iky * A["z"])[:, None] * g - A["y"][:, None] * dg  # noqa: E731
+    fv = lambda g, dg: A["x"][:, None] * dg - (ikx * A["z"])[:, None] * g  # noqa: E731
+    fw = lambda g, dg: (ikx * A["y"] - iky * A["x"])[:, None] * g  # noqa: E731
+    zc_in = z_c[1 : nz + 1]
+    zf_in = z_f[1:nz]
+    up = evaluate(fu, zc_in, dx, 0.5 * dy)
+    vp = evaluate(fv, zc_in, 0.5 * dx, dy)
+    wp = evaluate(fw, zf_in, 0.5 * dx, 0.5 * dy)
+    ss = ((up**2).sum() + (vp**2).sum() + (wp**2).sum()).item() / (up.numel() + vp.numel()
+                                                                   + wp.numel())
+    scale = urms / math.sqrt(ss) if ss > 0 else 0.0
+    f64 = dict(dtype=torch.float64, device=dev)
+    u = torch.zeros((nx + 1, ny + 2, nz + 2), **f64)
+    v = torch.zeros((nx + 2, ny + 1, nz + 2), **f64)
+    w = torch.zeros((nx + 2, ny + 2, nz + 1), **f64)
+    zeta = 2.0 * zc_in / Lz - 1.0
+    base = torch.as_tensor(1.5 * U_bulk * (1.0 - zeta**2), **f64)
     u[1 : nx + 1, 1 : ny + 1, 1 : nz + 1] = scale * up + base[None, None, :]
     v[1 : nx + 1, 1 : ny + 1, 1 : nz + 1] = scale * vp
     w[1 : nx + 1, 1 : ny + 1, 1:nz] = scale * wp

@@ -1,0 +1,65 @@
+"""Serving loop (slchannel_amd/serve.py): continuous batching by HBM footprint.
+
+Requests of different grid sizes and step counts are served by one batched
+ensemble under an HBM budget that admits only part of them at once, so runs
+join while others are mid-flight (their BDF1 bootstrap inside the others'
+BDF2 launches) and leave when done.  Each result must equal the same request
+run alone on the plain per-run path (SLChannelFlow.step_sl_bdf2) bit for
+bit: fields, pressure, bulk velocity, forcing and clamp count.
+"""
+
+import math
+
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _req(rid, nx, ny, nz, Re, steps, arrival=0.0):
+    from slchannel_amd.serve import Request
+
+    cfg = {"grid": {"nx": nx, "ny": ny, "nz": nz},
+           "domain": {"Lx": 4 * math.pi, "Ly": 2 * math.pi, "Lz": 2.0},
+           "flow": {"Re": Re, "U_bulk": 1.0, "gamma": 1.5},
+           "sl": {"interp_order": 6}}
+    return Request(rid=rid, config=cfg, n_steps=steps, seed=7000 + rid, arrival=arrival,
+                   kmax=4.0)
+
+
+def test_results_same_alone_and_in_the_mix(check):
+    from slchannel_amd.ensemble import member_bytes
+    from slchannel_amd.serve import Server, run_alone
+
+    reqs = [_req(0, 24, 16, 24, 2800.0, 7), _req(1, 32, 24, 16, 4000.0, 4),
+            _req(2, 24, 16, 24, 3500.0, 6, 0.01), _req(3, 16, 16, 32, 3000.0, 5, 0.02),
+            _req(4, 32, 24, 16, 5000.0, 3, 0.03)]
+    sizes = [member_bytes(r.config) for r in reqs]
+    # room for two or three at once: later requests wait for a finished run
+    budget = int(1.5 * (sorted(sizes)[-1] + sorted(sizes)[-2]))
+    srv = Server(hbm_budget=budget, keep_fields=True)
+    active_max = [0]
+    log = []
+
+    def note(msg):
+        log.append(msg)
+        active_max[0] = max(active_max[0], int(msg.split("active=")[1]))
+
+    results, metrics = srv.run(reqs, log=note)
+    check("every request served", sorted(results) == [0, 1, 2, 3, 4], f"{sorted(results)}")
+    check("admission bounded by the HBM budget", 1 < active_max[0] < len(reqs),
+          f"max concurrent={active_max[0]} log={log}")
+    joined_mid = any(r.admitted > 0 and r.admitted < max(x.finished for x in results.values())
+                     for r in results.values())
+    check("runs joined while others were mid-flight", joined_mid, "")
+    for r in reqs:
+        alone, _ = run_alone(r)
+        got = results[r.rid]
+        same = all(torch.equal(got.fields[k], alone.fields[k]) for k in "uvwp")
+        dev = max(((got.fields[k] - alone.fields[k]).abs().max().item()) for k in "uvwp")
+        check(f"request {r.rid}: mix == alone ({r.n_steps} steps)", same and
+              got.steps == alone.steps, f"max|d|={dev:.1e}")
+        check(f"request {r.rid}: u_bulk, forcing, clamps == alone",
+              got.stats[:3] == alone.stats[:3], f"{got.stats[:3]} vs {alone.stats[:3]}")
+    check("latency percentiles", metrics["p99_step_ms"] >= metrics["p50_step_ms"] > 0,
+          f"{metrics}")
