@@ -343,7 +343,7 @@ def ensemble_bench(args):
     ens, ids = build(64)
     for _ in range(W):
         ens.step()
-    tune = ens.autotune(reps=2)
+    tune, tune_grid = ens.autotune(reps=2)
     torch.cuda.synchronize()
     t = time.perf_counter()
     for _ in range(K):
@@ -390,7 +390,7 @@ def ensemble_bench(args):
         "eager_member_steps_per_s": round(args.members * K / eager_s, 2),
         "graph_member_steps_per_s": round(args.members * K / graph_s, 2),
         "grid_point_steps_per_s": round(points * K / min(eager_s, graph_s), 1),
-        "autotune_ms": tune,
+        "autotune_ms": {"tile_shape": tune, "grid_per_cu": tune_grid},
         "member0_identical_to_plain": bool(same0),
         "fp32_storage_rel_err_vs_fp64": {k: float(f"{v:.3e}") for k, v in err.items()},
         "roofline": None, "cpu_baseline": None,
@@ -402,24 +402,27 @@ def serve_bench(args):
     """Poisson arrivals (rate per second, seeded) of requests with grids from
     {64^3, 128x96x128, 256^3, 384x384x256}, Re ~ U[2800, 10000], N ~ U[20, 200]
     steps, order 6, dt for trajectory CFL 3; admission by HBM footprint under
-    the given budget.  The same requests are then run one at a time on the
-    plain per-run path."""
+    the given budget.  The plain per-run path is each request run alone
+    (SLChannelFlow), served first-come-first-served under the same arrival
+    times (its queue simulated from the measured stand-alone durations)."""
     import numpy as np
 
-    from slchannel_amd.serve import Server, poisson_requests, run_alone
+    from slchannel_amd.serve import Server, plain_fifo, poisson_requests, run_alone
 
     grids = ((64, 64, 64), (128, 96, 128), (256, 256, 256), (384, 384, 256))
     reqs = poisson_requests(args.requests, args.rate, grids, (2800.0, 10000.0), (20, 200),
                             seed=5000, order=args.order)
     srv = Server(hbm_budget=int(args.hbm_budget_gb * 1e9))
+    srv.warmup([r.config for r in reqs])  # server start-up: one initialiser per grid
     results, m = srv.run(reqs)
-    plain_s, plain_lat = 0.0, []
+    durations, plain_lat = [], []
     for r in reqs:
         res, el = run_alone(r, keep_fields=False)
-        plain_s += el
+        durations.append(el)
         plain_lat += res.latencies_ms
+    pf = plain_fifo(reqs, durations)
     total_steps = sum(r.n_steps for r in reqs)
-    plain = total_steps / plain_s
+    plain = pf["member_steps_per_s"]
     line = {
         "metric": "member-steps/s served (continuous batching, Poisson arrivals)",
         "value": round(m["member_steps_per_s"], 2), "unit": "member-steps/s", "n_gpus": 1,
@@ -430,11 +433,17 @@ def serve_bench(args):
                                f"{list(grids)}, Re U[2800,10000], N U[20,200]",
                    "hbm_budget_GB": args.hbm_budget_gb},
         "p50_step_ms": round(m["p50_step_ms"], 3), "p99_step_ms": round(m["p99_step_ms"], 3),
-        "busy_member_steps_per_s": round(m["member_steps_per_busy_s"], 2),
+        "p50_time_in_system_s": round(m["p50_time_in_system_s"], 3),
+        "p99_time_in_system_s": round(m["p99_time_in_system_s"], 3),
         "wall_s": round(m["wall_s"], 2),
-        "plain_per_run_member_steps_per_s": round(plain, 2),
+        "device_step_s": round(m["device_step_s"], 2),
+        "host_s": m["host_s"],
+        "plain_fifo_member_steps_per_s": round(plain, 2),
         "plain_p50_step_ms": round(float(np.percentile(plain_lat, 50)), 3),
         "plain_p99_step_ms": round(float(np.percentile(plain_lat, 99)), 3),
+        "plain_p50_time_in_system_s": round(pf["p50_time_in_system_s"], 3),
+        "plain_p99_time_in_system_s": round(pf["p99_time_in_system_s"], 3),
+        "plain_back_to_back_member_steps_per_s": round(total_steps / sum(durations), 2),
         "served_vs_plain": round(m["member_steps_per_s"] / plain, 3),
         "roofline": None, "cpu_baseline": None,
     }

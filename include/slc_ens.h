@@ -72,8 +72,10 @@ int slc_ens_stats(slc_ens* ens, int id, double* out, void* stream);
 int slc_ens_step(slc_ens* ens, void* stream);
 
 /* Options:
- * SLC_ENS_OPT_TILE       SL tile shape 0: 4x4x32, 1: 2x8x32, 2: 8x4x16
- * SLC_ENS_OPT_GRID       persistent workgroups per CU of the non-SL kernels (1..32)
+ * SLC_ENS_OPT_TILE       SL tile shape 0: 4x4x16 (24 KB LDS), 1: 4x4x32 (40 KB),
+ *                        2: 2x8x16 (24 KB)
+ * SLC_ENS_OPT_GRID       workgroups of the non-SL kernels: 0 = one per work unit
+ *                        (default), k = 1..32 persistent workgroups per CU
  * SLC_ENS_OPT_GRAPH      1: replay each step from a captured hipGraph (re-captured
  *                        after membership changes), 0: launch eagerly */
 #define SLC_ENS_OPT_TILE 1
@@ -83,10 +85,12 @@ int slc_ens_set_option(slc_ens* ens, int option, int value);
 int slc_ens_get_option(const slc_ens* ens, int option, int* value);
 
 /* Autotuner: times the SL passes (far + near, three components) of the
- * current members `reps` times for every tile shape and keeps the fastest.
- * Leaves every member's state unchanged (the passes only write the
- * predictor scratch and the feet buffer; clamp counters are cleared).
- * report (optional): "shape ms\n" lines. */
+ * current members `reps` times for every tile shape, then the other batched
+ * substeps that write only scratch for grid modes 0, 4 and 16, and keeps the
+ * fastest of each.  Leaves every member's state unchanged (the timed
+ * kernels only write the predictor scratch, U*, the feet and spectral
+ * buffers; clamp counters are cleared).  report (optional): "shape ms\n"
+ * and "grid<k> ms\n" lines. */
 int slc_ens_autotune(slc_ens* ens, int reps, void* stream, char* report, int64_t len);
 
 #ifdef __cplusplus

@@ -9,7 +9,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libslc_hip.so")
+# SLCHANNEL_LIB overrides the library file (e.g. a build with another tile shape)
+LIB_PATH = os.environ.get("SLCHANNEL_LIB") or os.path.join(HERE, "libslc_hip.so")
 
 # Every symbol include/slc.h and include/slc_ens.h declare (checked by the
 # CPU test suite).

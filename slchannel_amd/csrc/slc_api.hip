@@ -49,11 +49,15 @@ namespace {
 
 std::once_flag g_fft_once;
 
+// Table upload on the plan's creation stream (non-blocking: it neither waits
+// for nor delays work other callers have queued on the device).
 int upload(slc_plan* p, const std::vector<double>& h, const double** out) {
     void* d = nullptr;
     int rc = dev_alloc(p, h.size() * sizeof(double), &d);
     if (rc) return rc;
-    HIP_OK(hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                          p->init_stream));
+    HIP_OK(hipStreamSynchronize(p->init_stream));
     *out = static_cast<const double*>(d);
     return SLC_OK;
 }
@@ -184,6 +188,13 @@ int slc::plan_create(const slc_grid_desc* d, int device, bool spectral, slc_plan
     std::call_once(g_fft_once, [] { rocfft_setup(); });
     slc_plan* p = new slc_plan();
     p->device = device;
+    {
+        hipError_t e = hipStreamCreateWithFlags(&p->init_stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete p;
+            return fail(SLC_EHIP, "plan stream: %s", hipGetErrorString(e));
+        }
+    }
     const int nx = d->nx, ny = d->ny, nz = d->nz, P = d->order;
     p->nx = nx;
     p->ny = ny;
@@ -291,8 +302,9 @@ int slc::plan_create(const slc_grid_desc* d, int device, bool spectral, slc_plan
                 return rc;
             }
             p->at[c] = ArrivalTabs{(const double*)a, (const double*)b, (const int*)q};
-            hipError_t e = run_arrival_tabs(g, c, (double*)a, (double*)b, (int*)q, nullptr);
-            if (e == hipSuccess) e = hipDeviceSynchronize();
+            hipError_t e = run_arrival_tabs(g, c, (double*)a, (double*)b, (int*)q,
+                                            p->init_stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(p->init_stream);
             if (e != hipSuccess) {
                 slc_plan_destroy(p);
                 return fail(SLC_EHIP, "arrival tables: %s", hipGetErrorString(e));
@@ -349,6 +361,7 @@ int slc_plan_destroy(slc_plan* p) {
     if (p->fwd) rocfft_plan_destroy(p->fwd);
     if (p->inv) rocfft_plan_destroy(p->inv);
     if (p->info) rocfft_execution_info_destroy(p->info);
+    if (p->init_stream) (void)hipStreamDestroy(p->init_stream);
     for (auto& r : p->recs) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);

@@ -107,12 +107,13 @@ struct EnsLaunch {
 };
 
 // SL tile shapes the ensemble kernels are compiled for (autotuned):
-//   0: 4 x 4 x 32   1: 2 x 8 x 32   2: 8 x 4 x 16   (TI x TJ x TK)
+//   0: 4 x 4 x 16, 24 KB LDS   1: 4 x 4 x 32, 40 KB LDS   2: 2 x 8 x 16, 24 KB LDS
+//   (TI x TJ x TK arrival points per 256-thread workgroup)
 constexpr int kEnsTileShapes = 3;
 __host__ __device__ inline void ens_tile_shape(int s, int& ti, int& tj, int& tk) {
-    ti = s == 1 ? 2 : (s == 2 ? 8 : 4);
-    tj = s == 1 ? 8 : 4;
-    tk = s == 2 ? 16 : 32;
+    ti = s == 2 ? 2 : 4;
+    tj = s == 2 ? 8 : 4;
+    tk = s == 1 ? 32 : 16;
 }
 
 // launchers: `grid` is the persistent grid size (workgroups); the SL tile
@@ -141,5 +142,7 @@ hipError_t run_ens_bulk_partial(const EnsLaunch& L, int shift, unsigned grid, hi
 hipError_t run_ens_bulk_final(const EnsLaunch& L, int mode, unsigned grid, hipStream_t s);
 hipError_t run_ens_advance(const EnsLaunch& L, hipStream_t s);
 hipError_t run_ens_clear_ncl(const EnsLaunch& L, hipStream_t s);
+// *p = v, stream-ordered (no host memory involved)
+hipError_t run_ens_set_int(int* p, int v, hipStream_t s);
 
 }  // namespace slc

@@ -22,8 +22,27 @@ using namespace slc;
 namespace {
 
 // one member's host record
+// everything of a grid description but the viscosity: members with equal keys
+// can share one arena (tables, buffers) one after the other
+struct GeoKey {
+    int nx, ny, nz, stretching, order, traj_order, n_iters, top;
+    double Lx, Ly, Lz, gamma, tanh_g;
+    bool operator==(const GeoKey& o) const {
+        return nx == o.nx && ny == o.ny && nz == o.nz && stretching == o.stretching &&
+               order == o.order && traj_order == o.traj_order && n_iters == o.n_iters &&
+               top == o.top && Lx == o.Lx && Ly == o.Ly && Lz == o.Lz && gamma == o.gamma &&
+               tanh_g == o.tanh_g;
+    }
+};
+
+GeoKey geo_key(const slc_grid_desc* d) {
+    return GeoKey{d->nx,         d->ny, d->nz, d->stretching, d->order, d->traj_order,
+                  d->n_traj_iters, d->top_wall, d->Lx, d->Ly, d->Lz, d->gamma, d->tanh_g};
+}
+
 struct Member {
     int id = 0;
+    GeoKey key{};
     slc_plan* plan = nullptr;  // grid / Poisson / arrival tables, U*, feet, partials
     int nx = 0, ny = 0, nz = 0;
     double dt = 0.0, U_bulk = 1.0;
@@ -77,7 +96,7 @@ struct slc_ens {
     int order = 6;
     int storage = 64;
     int tile_shape = 0;
-    int grid_per_cu = 8;
+    int grid_per_cu = 0;  // 0: one workgroup per work unit; k: k persistent workgroups per CU
     bool use_graph = false;
     int ncu = 256;
     hipStream_t es = nullptr;
@@ -103,6 +122,17 @@ struct slc_ens {
     // graph
     hipGraphExec_t gexec = nullptr;
     int64_t class_bytes = 0;
+    // arenas of removed members, reused by later members of the same geometry
+    std::vector<Member*> pool;
+    size_t pool_max = 8;
+    // pinned, double-buffered staging of the device table (async uploads
+    // ordered on es behind the steps still reading the previous table)
+    struct Staging {
+        char* host = nullptr;
+        size_t cap = 0;
+        hipEvent_t done = nullptr;
+    } stage[2];
+    int stage_i = 0;
 };
 
 namespace {
@@ -201,13 +231,15 @@ int class_plans(slc_ens* e, SizeClass& c) {
     return SLC_OK;
 }
 
+// device buffer of at least n elements; grows geometrically (a reallocation
+// frees the old buffer, which waits for the device)
 template <typename T>
 int grow(T** p, size_t* cap, size_t n) {
     if (n <= *cap && *p) return SLC_OK;
     if (*p) (void)hipFree(*p);
     *p = nullptr;
+    size_t want = std::max<size_t>(std::max<size_t>(n, 16), 2 * *cap);
     *cap = 0;
-    size_t want = std::max<size_t>(n, 16);
     HIP_OK(hipMalloc(p, want * sizeof(T)));
     *cap = want;
     return SLC_OK;
@@ -223,8 +255,10 @@ unsigned tiles_of(const Member* m, int comp, int shape) {
 // rebuild classes, slots, the device member table, work prefixes and the
 // tile-to-member maps (outside any capture)
 int rebuild(slc_ens* e) {
-    drop_graph(e);
-    HIP_OK(hipStreamSynchronize(e->es));  // buffers below may be freed / reallocated
+    if (e->gexec) {
+        HIP_OK(hipStreamSynchronize(e->es));  // the captured graph may still run
+        drop_graph(e);
+    }
     const int n = (int)e->members.size();
     // size classes: keep existing ones (their plans), reset counts
     for (auto& c : e->classes) c.count = 0;
@@ -252,18 +286,20 @@ int rebuild(slc_ens* e) {
             continue;
         }
         if (c.count > c.cap) {
+            // hipFree waits for the device, so in-flight steps finish first
             if (c.planes) (void)hipFree(c.planes);
             if (c.dhat) (void)hipFree(c.dhat);
             if (c.cpw) (void)hipFree(c.cpw);
             c.planes = c.dhat = c.cpw = nullptr;
+            const int want = std::max(c.count, c.cap + (c.cap + 1) / 2);
             c.cap = 0;
             const long nky = c.ny / 2 + 1;
             const size_t per = (size_t)c.nx * c.ny * c.nz;
             const size_t perk = (size_t)c.nx * nky * c.nz;
-            HIP_OK(hipMalloc(&c.planes, per * c.count * sizeof(double)));
-            HIP_OK(hipMalloc(&c.dhat, 2 * perk * c.count * sizeof(double)));
-            HIP_OK(hipMalloc(&c.cpw, perk * c.count * sizeof(double)));
-            c.cap = c.count;
+            HIP_OK(hipMalloc(&c.planes, per * want * sizeof(double)));
+            HIP_OK(hipMalloc(&c.dhat, 2 * perk * want * sizeof(double)));
+            HIP_OK(hipMalloc(&c.cpw, perk * want * sizeof(double)));
+            c.cap = want;
         }
         e->class_bytes += (int64_t)c.cap * spectral_bytes(c.nx, c.ny, c.nz);
         if (int rc = class_plans(e, c)) return rc;
@@ -338,14 +374,39 @@ int rebuild(slc_ens* e) {
     if (n > 0) {
         if (int rc = grow(&e->d_mem, &e->d_mem_cap, (size_t)n)) return rc;
         if (int rc = grow(&e->d_start, &e->d_start_cap, e->h_start.size())) return rc;
-        HIP_OK(hipMemcpy(e->d_mem, tab.data(), n * sizeof(EnsMember), hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(e->d_start, e->h_start.data(), e->h_start.size() * sizeof(unsigned),
-                         hipMemcpyHostToDevice));
-        for (int k = 0; k < 3; ++k) {
+        for (int k = 0; k < 3; ++k)
             if (int rc = grow(&e->d_umap[k], &e->d_umap_cap[k], umap[k].size())) return rc;
-            HIP_OK(hipMemcpy(e->d_umap[k], umap[k].data(), umap[k].size() * sizeof(int),
-                             hipMemcpyHostToDevice));
+        // serialise into the pinned staging buffer not used by the last upload
+        const size_t sz_mem = n * sizeof(EnsMember);
+        const size_t sz_start = e->h_start.size() * sizeof(unsigned);
+        size_t total = sz_mem + sz_start;
+        for (int k = 0; k < 3; ++k) total += umap[k].size() * sizeof(int);
+        auto& st = e->stage[e->stage_i];
+        e->stage_i ^= 1;
+        if (st.done) HIP_OK(hipEventSynchronize(st.done));
+        if (total > st.cap) {
+            const size_t want = std::max(total, 2 * st.cap) + 4096;
+            if (st.host) (void)hipHostFree(st.host);
+            st.host = nullptr;
+            st.cap = 0;
+            HIP_OK(hipHostMalloc((void**)&st.host, want, hipHostMallocDefault));
+            st.cap = want;
         }
+        if (!st.done) HIP_OK(hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
+        size_t off = 0;
+        std::memcpy(st.host + off, tab.data(), sz_mem);
+        HIP_OK(hipMemcpyAsync(e->d_mem, st.host + off, sz_mem, hipMemcpyHostToDevice, e->es));
+        off += sz_mem;
+        std::memcpy(st.host + off, e->h_start.data(), sz_start);
+        HIP_OK(hipMemcpyAsync(e->d_start, st.host + off, sz_start, hipMemcpyHostToDevice, e->es));
+        off += sz_start;
+        for (int k = 0; k < 3; ++k) {
+            const size_t sz = umap[k].size() * sizeof(int);
+            std::memcpy(st.host + off, umap[k].data(), sz);
+            HIP_OK(hipMemcpyAsync(e->d_umap[k], st.host + off, sz, hipMemcpyHostToDevice, e->es));
+            off += sz;
+        }
+        HIP_OK(hipEventRecord(st.done, e->es));
     }
     e->dirty = false;
     return SLC_OK;
@@ -363,9 +424,10 @@ EnsLaunch launch(const slc_ens* e, int kind, int comp = -1) {
 }
 
 unsigned pgrid(const slc_ens* e, int kind) {
+    const unsigned t = e->totals[kind] ? e->totals[kind] : 1u;
+    if (e->grid_per_cu == 0) return t;
     const unsigned cap = (unsigned)(e->ncu * e->grid_per_cu);
-    const unsigned t = e->totals[kind];
-    return t < cap ? (t ? t : 1u) : cap;
+    return t < cap ? t : cap;
 }
 
 int fft_all(slc_ens* e, bool forward) {
@@ -493,7 +555,12 @@ int slc_ens_destroy(slc_ens* e) {
     if (e->es) (void)hipStreamSynchronize(e->es);
     drop_graph(e);
     for (Member* m : e->members) free_member(m);
+    for (Member* m : e->pool) free_member(m);
     for (auto& c : e->classes) free_class(c);
+    for (auto& st : e->stage) {
+        if (st.host) (void)hipHostFree(st.host);
+        if (st.done) (void)hipEventDestroy(st.done);
+    }
     if (e->d_mem) (void)hipFree(e->d_mem);
     if (e->d_start) (void)hipFree(e->d_start);
     for (int k = 0; k < 3; ++k)
@@ -524,19 +591,28 @@ int slc_ens_add(slc_ens* e, const slc_grid_desc* d, double dt, double U_bulk, in
     if (d->nz < 3) return fail(SLC_EINVAL, "ensemble members need nz >= 3");
     if (zsolve_cols(0, d->nz) < 1) return fail(SLC_EINVAL, "nz=%d too large for the z-solve", d->nz);
     DeviceGuard dg(e->device);
-    Member* m = new Member();
-    int rc = plan_create(d, e->device, false, &m->plan);
-    if (rc) {
-        delete m;
-        return rc;
-    }
-    m->nx = d->nx;
-    m->ny = d->ny;
-    m->nz = d->nz;
-    m->dt = dt;
-    m->U_bulk = U_bulk;
-    void* p = nullptr;
-    const int nx = m->nx, ny = m->ny, nz = m->nz;
+    const GeoKey key = geo_key(d);
+    Member* m = nullptr;
+    int rc = SLC_OK;
+    for (size_t i = 0; i < e->pool.size(); ++i)
+        if (e->pool[i]->key == key) {  // reuse a released arena of this geometry
+            m = e->pool[i];
+            e->pool.erase(e->pool.begin() + i);
+            break;
+        }
+    if (!m) {
+        m = new Member();
+        m->key = key;
+        rc = plan_create(d, e->device, false, &m->plan);
+        if (rc) {
+            delete m;
+            return rc;
+        }
+        m->nx = d->nx;
+        m->ny = d->ny;
+        m->nz = d->nz;
+        void* p = nullptr;
+        const int nx = m->nx, ny = m->ny, nz = m->nz;
 #define MALLOC(bytes, dst, T)                    \
     do {                                         \
         if ((rc = m_alloc(m, (bytes), &p))) {    \
@@ -545,21 +621,28 @@ int slc_ens_add(slc_ens* e, const slc_grid_desc* d, double dt, double U_bulk, in
         }                                        \
         dst = static_cast<T*>(p);                \
     } while (0)
-    for (int s = 0; s < kFieldSets; ++s)
-        for (int c = 0; c < 3; ++c) MALLOC(8 * field_elems(c, nx, ny, nz), m->fld[s][c], double);
-    MALLOC(8 * p_elems(nx, ny, nz), m->p, double);
-    if (e->storage == 32)
-        for (int c = 0; c < 3; ++c) {
-            MALLOC(4 * field_elems(c, nx, ny, nz), m->f32n[c], float);
-            MALLOC(4 * field_elems(c, nx, ny, nz), m->f32m[c], float);
-        }
-    MALLOC(8 * 18 * (size_t)(nz + 2), m->ztab, double);
-    MALLOC(8 * 4, m->scal, double);
-    MALLOC(8 * ES_COUNT, m->stats, double);
-    MALLOC(8, m->ncl, unsigned long long);
-    MALLOC(sizeof(int), m->state, int);
+        for (int s = 0; s < kFieldSets; ++s)
+            for (int c = 0; c < 3; ++c)
+                MALLOC(8 * field_elems(c, nx, ny, nz), m->fld[s][c], double);
+        MALLOC(8 * p_elems(nx, ny, nz), m->p, double);
+        if (e->storage == 32)
+            for (int c = 0; c < 3; ++c) {
+                MALLOC(4 * field_elems(c, nx, ny, nz), m->f32n[c], float);
+                MALLOC(4 * field_elems(c, nx, ny, nz), m->f32m[c], float);
+            }
+        MALLOC(8 * 18 * (size_t)(nz + 2), m->ztab, double);
+        MALLOC(8 * 4, m->scal, double);
+        MALLOC(8 * ES_COUNT, m->stats, double);
+        MALLOC(8, m->ncl, unsigned long long);
+        MALLOC(sizeof(int), m->state, int);
 #undef MALLOC
-    // zero fields, scratch, counters; state: a = 0, b = 1, c = 2, BDF1 next
+    }
+    m->plan->g.nu = d->nu;  // the only per-run grid parameter
+    m->dt = dt;
+    m->U_bulk = U_bulk;
+    const int nx = m->nx, ny = m->ny, nz = m->nz;
+    // zero fields, scratch, counters; state: a = 0, b = 1, c = 2, BDF1 next.
+    // Stream-ordered on es: after any step still using a reused arena.
     hipStream_t s0 = e->es;
     hipError_t err = hipSuccess;
     for (int s = 0; s < kFieldSets && err == hipSuccess; ++s)
@@ -570,15 +653,13 @@ int slc_ens_add(slc_ens* e, const slc_grid_desc* d, double dt, double U_bulk, in
     if (err == hipSuccess) err = hipMemsetAsync(m->stats, 0, 8 * ES_COUNT, s0);
     if (err == hipSuccess) err = hipMemsetAsync(m->ncl, 0, 8, s0);
     m->host_state = ens_state(0, 1, 2, 0);
-    if (err == hipSuccess)
-        err = hipMemcpyAsync(m->state, &m->host_state, sizeof(int), hipMemcpyHostToDevice, s0);
+    if (err == hipSuccess) err = run_ens_set_int(m->state, m->host_state, s0);
     // z-solve factors: mode 0 (BDF1, dt), mode 1 (BDF2, 2dt/3); theta = 1
     for (int mode = 0; mode < 2 && err == hipSuccess; ++mode)
         for (int c = 0; c < 3 && err == hipSuccess; ++c)
             err = run_zfactor(m->plan->g, c, mode == 0 ? dt : (2.0 / 3.0) * dt, 1.0,
                               m->plan->d_dzc, m->plan->d_dzf,
                               m->ztab + (size_t)(mode * 9 + c * 3) * (nz + 2), s0);
-    if (err == hipSuccess) err = hipStreamSynchronize(s0);
     if (err != hipSuccess) {
         free_member(m);
         return fail(SLC_EHIP, "ensemble member init: %s", hipGetErrorString(err));
@@ -595,10 +676,16 @@ int slc_ens_remove(slc_ens* e, int id) {
     DeviceGuard dg(e->device);
     for (size_t i = 0; i < e->members.size(); ++i)
         if (e->members[i]->id == id) {
-            HIP_OK(hipStreamSynchronize(e->es));
-            free_member(e->members[i]);
+            // keep the arena for a later member of the same geometry; the
+            // oldest pooled arena is freed beyond pool_max (hipFree waits
+            // for the device)
+            e->pool.push_back(e->members[i]);
             e->members.erase(e->members.begin() + i);
             e->dirty = true;
+            if (e->pool.size() > e->pool_max) {
+                free_member(e->pool.front());
+                e->pool.erase(e->pool.begin());
+            }
             return SLC_OK;
         }
     return fail(SLC_EINVAL, "no member with id %d", id);
@@ -640,8 +727,7 @@ int slc_ens_set_fields(slc_ens* e, int id, const double* u, const double* v, con
         HIP_OK(hipMemsetAsync(m->p, 0, 8 * p_elems(m->nx, m->ny, m->nz), e->es));
     // restart semantics: next step is a BDF1 bootstrap from buffer set a
     m->host_state = ens_state(a, (a + 1) % 3, (a + 2) % 3, 0);
-    HIP_OK(hipMemcpyAsync(m->state, &m->host_state, sizeof(int), hipMemcpyHostToDevice, e->es));
-    HIP_OK(hipStreamSynchronize(e->es));  // host_state is a stack-stable member; keep it simple
+    HIP_OK(run_ens_set_int(m->state, m->host_state, e->es));
     return leave(e, stream);
 }
 
@@ -720,7 +806,7 @@ int slc_ens_set_option(slc_ens* e, int option, int value) {
             }
             return SLC_OK;
         case SLC_ENS_OPT_GRID:
-            if (value < 1 || value > 32) return fail(SLC_EINVAL, "grid per CU %d out of range", value);
+            if (value < 0 || value > 32) return fail(SLC_EINVAL, "grid per CU %d out of range", value);
             if (value != e->grid_per_cu) {
                 e->grid_per_cu = value;
                 drop_graph(e);
@@ -787,12 +873,52 @@ int slc_ens_autotune(slc_ens* e, int reps, void* stream, char* report, int64_t l
             best = shape;
         }
     }
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     e->tile_shape = best;
     e->dirty = true;
+    if (rc == SLC_OK) rc = rebuild(e);
+    // grid of the other batched kernels: time the substeps that only write
+    // scratch (ghost fill of V^n, U*, z-solves of the predictor buffer,
+    // divergence planes, Thomas)
+    int best_grid = e->grid_per_cu;
+    float best_gms = 1e30f;
+    const int grids[3] = {0, 4, 16};
+    for (int gi = 0; gi < 3 && rc == SLC_OK; ++gi) {
+        e->grid_per_cu = grids[gi];
+        auto substeps = [&]() -> int {
+            hipStream_t s = e->es;
+            HIP_OK(run_ens_bc3(launch(e, EK_BC3), 0, pgrid(e, EK_BC3), s));
+            HIP_OK(run_ens_ustar(launch(e, EK_ELEM), pgrid(e, EK_ELEM), s));
+            for (int c = 0; c < 3; ++c)
+                HIP_OK(run_ens_zsolve(launch(e, EK_ZS0 + c), c, e->zs_lds[c],
+                                      pgrid(e, EK_ZS0 + c), s));
+            HIP_OK(run_ens_div_planes(launch(e, EK_PLANE), pgrid(e, EK_PLANE), s));
+            HIP_OK(run_ens_thomas(launch(e, EK_THOMAS), pgrid(e, EK_THOMAS), s));
+            return SLC_OK;
+        };
+        if ((rc = substeps())) break;
+        (void)hipEventRecord(t0, e->es);
+        for (int r = 0; r < reps && rc == SLC_OK; ++r) rc = substeps();
+        (void)hipEventRecord(t1, e->es);
+        if (rc) break;
+        if (hipEventSynchronize(t1) != hipSuccess) {
+            rc = fail(SLC_EHIP, "autotune timing failed");
+            break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, t0, t1);
+        ms /= (float)reps;
+        char line[64];
+        snprintf(line, sizeof line, "grid%d %.4f\n", grids[gi], ms);
+        rep += line;
+        if (ms < best_gms) {
+            best_gms = ms;
+            best_grid = grids[gi];
+        }
+    }
+    e->grid_per_cu = best_grid;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
     if (rc) return rc;
-    if (int r2 = rebuild(e)) return r2;
     // the timed passes accumulated clamp counts: clear them
     HIP_OK(run_ens_clear_ncl(launch(e, EK_ONE), e->es));
     if (report && len > 0) {

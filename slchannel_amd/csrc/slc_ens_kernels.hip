@@ -283,4 +283,11 @@ hipError_t run_ens_clear_ncl(const EnsLaunch& L, hipStream_t s) {
     return hipGetLastError();
 }
 
+__global__ void ens_set_int_kernel(int* p, int v) { *p = v; }
+
+hipError_t run_ens_set_int(int* p, int v, hipStream_t s) {
+    hipLaunchKernelGGL(ens_set_int_kernel, dim3(1), dim3(1), 0, s, p, v);
+    return hipGetLastError();
+}
+
 }  // namespace slc

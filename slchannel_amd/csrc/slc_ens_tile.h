@@ -9,13 +9,12 @@
 
 namespace slc {
 
-constexpr int kEnsLds = 40960;
-
 // One workgroup per tile: unit = xcd-ordered block index, its member from the
 // per-unit member map (the persistent chunk walk of the other batched kernels
 // keeps ~25 more registers live here and halves the occupancy).
-template <typename T, int C, int P, int PASS, int TI, int TJ, int TK>
-__global__ __launch_bounds__(kSB) void ens_tile_kernel(const EnsMember* __restrict__ mem,
+template <typename T, int C, int P, int PASS, int TI, int TJ, int TK, int kEnsLds, int WAVES>
+__global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(WAVES))) void
+ens_tile_kernel(const EnsMember* __restrict__ mem,
                                                        const unsigned* __restrict__ start,
                                                        const int* __restrict__ umap) {
     __shared__ T lds[kEnsLds / sizeof(T)];
@@ -60,16 +59,16 @@ template <typename T, int C, int P, int PASS>
 inline void launch_tile_shape(const EnsLaunch& L, int shape, unsigned grid, hipStream_t s) {
     switch (shape) {
         case 1:
-            hipLaunchKernelGGL((ens_tile_kernel<T, C, P, PASS, 2, 8, 32>), dim3(grid), dim3(kSB), 0,
-                               s, L.mem, L.start, L.umap);
+            hipLaunchKernelGGL((ens_tile_kernel<T, C, P, PASS, 4, 4, 32, 40960, 1>), dim3(grid),
+                               dim3(kSB), 0, s, L.mem, L.start, L.umap);
             break;
         case 2:
-            hipLaunchKernelGGL((ens_tile_kernel<T, C, P, PASS, 8, 4, 16>), dim3(grid), dim3(kSB), 0,
-                               s, L.mem, L.start, L.umap);
+            hipLaunchKernelGGL((ens_tile_kernel<T, C, P, PASS, 2, 8, 16, 24576, 4>), dim3(grid),
+                               dim3(kSB), 0, s, L.mem, L.start, L.umap);
             break;
         default:
-            hipLaunchKernelGGL((ens_tile_kernel<T, C, P, PASS, 4, 4, 32>), dim3(grid), dim3(kSB), 0,
-                               s, L.mem, L.start, L.umap);
+            hipLaunchKernelGGL((ens_tile_kernel<T, C, P, PASS, 4, 4, 16, 24576, 4>), dim3(grid),
+                               dim3(kSB), 0, s, L.mem, L.start, L.umap);
     }
 }
 

@@ -49,6 +49,7 @@ struct slc_plan {
     using PoissonTabs = slc::PoissonTabs;
     using ArrivalTabs = slc::ArrivalTabs;
     int device = 0;
+    hipStream_t init_stream = nullptr;  // non-blocking stream of the table uploads
     Grid g{};
     PoissonTabs pt{};
     int nx = 0, ny = 0, nz = 0, nky = 0;

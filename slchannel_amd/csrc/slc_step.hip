@@ -4,8 +4,26 @@
 
 namespace slc {
 
+// tile shape of the step kernels: 4 x 4 x 16 arrival points (one per
+// thread), 24 KB of LDS, at least 4 waves per SIMD (measured at 256^3 order 6
+// against 4x4x32 / 2x8x16 / 8x4x8 / 4x8x8 tiles and 20-40 KB budgets).
+// Compile-time overridable with -DSLC_TILE_TI=... for tile-shape sweeps.
+#ifndef SLC_TILE_TI
+#define SLC_TILE_TI 4
+#define SLC_TILE_TJ 4
+#define SLC_TILE_TK 16
+#define SLC_TILE_LDS 24576
+#define SLC_TILE_WAVES 4
+#endif
+
+#ifndef SLC_TILE_WAVES
+#define SLC_TILE_ATTR
+#else
+#define SLC_TILE_ATTR __attribute__((amdgpu_waves_per_eu(SLC_TILE_WAVES)))
+#endif
+
 template <typename T, int C, int P, int MODE, int TI, int TJ, int TK, int LDS_BYTES>
-__global__ __launch_bounds__(kSB) void sl_tile_kernel(
+__global__ __launch_bounds__(kSB) SLC_TILE_ATTR void sl_tile_kernel(
     Grid g, ArrivalTabs A, TrajVel V, const double* __restrict__ F,
     const double* __restrict__ Fm, const T* __restrict__ Sn, const T* __restrict__ Sm,
     double dt, double* __restrict__ out, Feet ft, unsigned long long* nclamp) {
@@ -125,8 +143,7 @@ hipError_t run_to_f32(const Grid& g, float* const o[3], const double* const a[3]
     return hipGetLastError();
 }
 
-// tile shape of the step kernels: 4 x 4 x 32 arrival points, 40 KB of LDS
-constexpr int kTI = 4, kTJ = 4, kTK = 32, kLDS = 40960;
+constexpr int kTI = SLC_TILE_TI, kTJ = SLC_TILE_TJ, kTK = SLC_TILE_TK, kLDS = SLC_TILE_LDS;
 
 struct StepArgs {
     const double* F;

@@ -214,24 +214,28 @@ __device__ __forceinline__ void sl_tile_body(
     const T* __restrict__ S = (MODE == MODE_FAR) ? Sm : Sn;
     const SBox b = s_block_box(l, sh);
     const int BX = b.xmax - b.xmin + P, BY = b.ymax - b.ymin + P, BZ = b.kmax - b.kmin + P;
-    const bool fits = b.xmin <= b.xmax && (long)BX * BY * BZ <= CAP;
+    // LDS: the box (z fastest), then one int per (x, y) row: its global base
+    const int rows = BX * BY;
+    const int nbox = rows * BZ;
+    const bool fits = b.xmin <= b.xmax &&
+                      (long)BX * BY * BZ * (long)sizeof(T) + (long)BX * BY * 4 <= LDS_BYTES;
     if (fits) {
-        // flat element loop (z fastest: coalesced rows); the quotients e/BZ
-        // and row/BY come from float reciprocals -- exact here because
-        // e < CAP < 2^13 keeps every fractional part >= 0.5/BZ from an integer
-        const int rows = BX * BY;
-        const float rbz = 1.0f / (float)BZ, rby = 1.0f / (float)BY;
-        for (int e = threadIdx.x; e < rows * BZ; e += kSB) {
+        int* rowbase = reinterpret_cast<int*>(lds + nbox);
+        for (int r = threadIdx.x; r < rows; r += kSB) {
+            const int x = r / BY, y = r - (r / BY) * BY;
+            rowbase[r] = node_addr<C>(g, wrap_idx(b.xmin + x, g.nx),
+                                      wrap_idx(b.ymin + y, g.ny), b.kmin);
+        }
+        __syncthreads();
+        // flat element loop (z fastest: coalesced rows); the quotient e/BZ
+        // comes from a float reciprocal, made exact by one integer
+        // correction each way (the estimate is within 1 of the quotient)
+        const float rbz = 1.0f / (float)BZ;
+        for (int e = threadIdx.x; e < nbox; e += kSB) {
             int row = (int)(((float)e + 0.5f) * rbz);
-            row += (row + 1) * BZ <= e ? 1 : 0;  // exact even if the reciprocal
-            row -= row * BZ > e ? 1 : 0;         // were off by an ulp
-            const int zz = e - row * BZ;
-            int x = (int)(((float)row + 0.5f) * rby);
-            x += (x + 1) * BY <= row ? 1 : 0;
-            x -= x * BY > row ? 1 : 0;
-            const int y = row - x * BY;
-            const int gx = wrap_idx(b.xmin + x, g.nx), gy = wrap_idx(b.ymin + y, g.ny);
-            lds[e] = S[node_addr<C>(g, gx, gy, b.kmin + zz)];
+            row += (row + 1) * BZ <= e ? 1 : 0;
+            row -= row * BZ > e ? 1 : 0;
+            lds[e] = S[rowbase[row] + (e - row * BZ)];
         }
     }
     __syncthreads();

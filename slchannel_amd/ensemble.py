@@ -187,16 +187,19 @@ class Ensemble:
         return v.value
 
     def autotune(self, reps=3):
-        """Time the SL passes for every tile shape; keep the fastest.
-        Returns {shape: ms}."""
+        """Time the SL passes for every tile shape and the other substeps for
+        every grid mode; keep the fastest.  Returns ({shape: ms}, {grid: ms})."""
         buf = ctypes.create_string_buffer(1024)
         _lib.check(self._lib.slc_ens_autotune(self._h, int(reps), self._stream(), buf, 1024),
                    "slc_ens_autotune")
-        out = {}
+        shapes, grids = {}, {}
         for line in buf.value.decode().splitlines():
             s, ms = line.split()
-            out[int(s)] = float(ms)
-        return out
+            if s.startswith("grid"):
+                grids[int(s[4:])] = float(ms)
+            else:
+                shapes[int(s)] = float(ms)
+        return shapes, grids
 
 
 _INIT_CACHE = {}

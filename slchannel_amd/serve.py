@@ -11,10 +11,11 @@ with BDF1 inside the same launches in which the others take BDF2 steps, and
 a run that has done its steps is copied out and its memory released, so a
 waiting request can take its place.
 
-Reported per run: admission and completion times, every step's latency (the
-wall time of the batched step it took part in, launch to completion), and
-optionally its final fields.  Aggregates: p50 / p99 step latency and member
-steps per second over the busy period.
+Reported per run: arrival, admission and completion times, every step's
+latency (the device time of the batched step it took part in, from
+hipEvents), and optionally its final fields.  Aggregates: p50 / p99 step
+latency, p50 / p99 time in system, member steps per second over the span
+from the first admission to the last completion.
 
 The reference has no serving layer (one run per process, solver.py:1280-1500).
 """
@@ -46,6 +47,7 @@ class Request:
 @dataclass
 class Result:
     rid: int
+    arrival: float = 0.0
     admitted: float = 0.0
     finished: float = 0.0
     dt: float = 0.0
@@ -82,37 +84,77 @@ class Server:
         self.max_members = int(max_members)
         self.keep_fields = keep_fields
         self.storage = storage
+        # admissions (initial fields, dt) run on their own stream so they
+        # overlap the batched steps instead of queueing behind them
+        self.side = torch.cuda.Stream(self.device)
+
+    def warmup(self, configs):
+        """Server start-up for a grid catalogue: builds the per-grid initialiser
+        (its rocFFT plans) of every distinct grid before requests arrive."""
+        seen = set()
+        for cfg in configs:
+            c = _config_grid(cfg)
+            key = (c["nx"], c["ny"], c["nz"], c["Lx"], c["Ly"], c["Lz"], c["gamma"],
+                   c["stretching"], c["top"], c["order"])
+            if key in seen:
+                continue
+            seen.add(key)
+            with torch.cuda.stream(self.side):
+                initial_fields(cfg, 0, device=self.device)
+        torch.cuda.synchronize(self.device)
 
     def fits(self, req):
         need = member_bytes(req.config, self.storage)
         return need <= self.budget - self.ens.bytes() and len(self.ens) < self.max_members
 
-    def run(self, requests, log=None):
+    def run(self, requests, log=None, inflight=3):
         """Serve `requests` (arrival times relative to the start); returns
-        (results by rid, metrics dict)."""
+        (results by rid, metrics dict).  Steps are pipelined: the host keeps
+        at most `inflight` batched steps queued ahead of the device and only
+        synchronises to retire a finished run; each step's latency is its
+        device time from hipEvents."""
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         for r in pending:
             if member_bytes(r.config, self.storage) > self.budget:
                 raise ValueError(f"request {r.rid} alone exceeds the HBM budget")
         results = {}
         active = {}  # member id -> (request, result)
+        queued = []  # (start event, end event, results stepped)
+        retiring = []  # (event after the copies out, request, result)
         t0 = time.perf_counter()
-        busy = 0.0
         n_member_steps = 0
         first_admit = None
+        host = {"admit_s": 0.0, "step_s": 0.0, "retire_s": 0.0, "wait_s": 0.0}
+
+        step_ms = []
+
+        def drain(keep):
+            while len(queued) > keep:
+                ev0, ev1, stepped = queued.pop(0)
+                ev1.synchronize()
+                ms = ev0.elapsed_time(ev1)
+                step_ms.append(ms)
+                for res in stepped:
+                    res.latencies_ms.append(ms)
+
         while pending or active:
             now = time.perf_counter() - t0
-            # admission: FIFO over the requests that have arrived
-            while pending and pending[0].arrival <= now and self.fits(pending[0]):
+            # admission: FIFO over the requests that have arrived, one per
+            # iteration so the device always has a queued step meanwhile
+            if pending and pending[0].arrival <= now and self.fits(pending[0]):
                 req = pending.pop(0)
-                fields, dt = _prepare(req, self.device)
-                mid = self.ens.add(req.config, dt, fields=fields)
+                ta = time.perf_counter()
+                with torch.cuda.stream(self.side):
+                    fields, dt = _prepare(req, self.device)
+                    mid = self.ens.add(req.config, dt, fields=fields)
+                host["admit_s"] += time.perf_counter() - ta
                 res = Result(rid=req.rid, admitted=time.perf_counter() - t0, dt=dt)
+                res.arrival = req.arrival
                 active[mid] = (req, res)
                 first_admit = res.admitted if first_admit is None else first_admit
                 if log:
-                    log(f"admit {req.rid} {_config_grid(req.config)['nx']}x"
-                        f"{_config_grid(req.config)['ny']}x{_config_grid(req.config)['nz']} "
+                    c = _config_grid(req.config)
+                    log(f"admit {req.rid} {c['nx']}x{c['ny']}x{c['nz']} "
                         f"steps={req.n_steps} active={len(active)}")
             if not active:
                 if pending:
@@ -120,40 +162,62 @@ class Server:
                     if wait > 0:
                         time.sleep(min(wait, 0.05))
                 continue
-            torch.cuda.synchronize(self.device)
+            tw = time.perf_counter()
+            drain(inflight - 1)
             ts = time.perf_counter()
+            host["wait_s"] += ts - tw
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            ev0.record()
             self.ens.step()
-            torch.cuda.synchronize(self.device)
-            te = time.perf_counter()
-            lat = (te - ts) * 1e3
-            busy += te - ts
+            ev1.record()
+            host["step_s"] += time.perf_counter() - ts
+            queued.append((ev0, ev1, [res for _, res in active.values()]))
             n_member_steps += len(active)
             done = []
             for mid, (req, res) in active.items():
-                res.latencies_ms.append(lat)
                 res.steps += 1
                 if res.steps >= req.n_steps:
                     done.append(mid)
+            tr = time.perf_counter()
             for mid in done:
                 req, res = active.pop(mid)
+                # copies are stream-ordered behind the member's last step;
+                # the arena is reused only by work queued after them
                 if self.keep_fields:
                     res.fields = self.ens.get_fields(mid)
-                res.stats = self.ens.stats(mid).cpu().tolist()
-                res.finished = time.perf_counter() - t0
+                res.stats = self.ens.stats(mid)
+                ev = torch.cuda.Event()
+                ev.record()
+                retiring.append((ev, req, res))
                 self.ens.remove(mid)
+            while retiring and retiring[0][0].query():
+                ev, req, res = retiring.pop(0)
+                res.finished = time.perf_counter() - t0
+                res.stats = res.stats.cpu().tolist()
                 results[req.rid] = res
+            host["retire_s"] += time.perf_counter() - tr
+        for ev, req, res in retiring:
+            ev.synchronize()
+            res.finished = time.perf_counter() - t0
+            res.stats = res.stats.cpu().tolist()
+            results[req.rid] = res
+        drain(0)
         t_end = time.perf_counter() - t0
         lats = np.array([x for r in results.values() for x in r.latencies_ms])
         span = t_end - (first_admit or 0.0)
+        in_sys = np.array([r.finished - r.arrival for r in results.values()])
         metrics = {
             "requests": len(results),
             "member_steps": int(n_member_steps),
             "p50_step_ms": float(np.percentile(lats, 50)) if lats.size else math.nan,
             "p99_step_ms": float(np.percentile(lats, 99)) if lats.size else math.nan,
             "member_steps_per_s": n_member_steps / span if span > 0 else math.nan,
-            "member_steps_per_busy_s": n_member_steps / busy if busy > 0 else math.nan,
+            "p50_time_in_system_s": float(np.percentile(in_sys, 50)) if in_sys.size else math.nan,
+            "p99_time_in_system_s": float(np.percentile(in_sys, 99)) if in_sys.size else math.nan,
             "wall_s": t_end,
-            "busy_s": busy,
+            "device_step_s": float(sum(ms for ms in step_ms)) / 1e3,
+            "host_s": {k: round(v, 3) for k, v in host.items()},
         }
         return results, metrics
 
@@ -198,3 +262,24 @@ def run_alone(req, device=None, keep_fields=True):
         res.fields = {"u": flow.u, "v": flow.v, "w": flow.w, "p": flow.p}
     res.stats = [ub.item(), fo.item(), float(flow.sl.n_clamped_last.item()), float(res.steps)]
     return res, el
+
+
+def plain_fifo(requests, durations):
+    """The plain per-run path under the same arrivals: requests served one at
+    a time in arrival order, each taking its measured stand-alone duration.
+    Returns member steps per second over the span and the p50 / p99 time in
+    system."""
+    t = 0.0
+    first = None
+    in_sys = []
+    steps = 0
+    for r, d in sorted(zip(requests, durations), key=lambda x: (x[0].arrival, x[0].rid)):
+        start = max(t, r.arrival)
+        first = start if first is None else first
+        t = start + d
+        in_sys.append(t - r.arrival)
+        steps += r.n_steps
+    span = t - (first or 0.0)
+    return {"member_steps_per_s": steps / span if span > 0 else math.nan,
+            "p50_time_in_system_s": float(np.percentile(in_sys, 50)),
+            "p99_time_in_system_s": float(np.percentile(in_sys, 99))}

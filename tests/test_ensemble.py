@@ -110,6 +110,17 @@ def test_join_and_leave_between_steps(check):
         _compare(check, f"after join {n + 1}: a", ens, ia, fa, ua)
         _compare(check, f"after join {n + 1}: b", ens, ib, fb, ubb)
     check("membership", len(ens) == 2, f"{len(ens)}")
+    # d has c's geometry (other Re, dt, seed): it reuses c's released arena
+    cd = _cfg(24, 16, 24, 5000.0)
+    fd = _single(cd, 21)
+    ida = ens.add(cd, 0.015, fields=(fd.u, fd.v, fd.w, fd.p))
+    for n in range(3):
+        ens.step()
+        fa.step_sl_bdf2(dta)
+        fb.step_sl_bdf2(dtb)
+        ud = fd.step_sl_bdf2(0.015)[0]
+        _compare(check, f"reused arena {n + 1}: d", ens, ida, fd, ud)
+    _compare(check, "reused arena: a still exact", ens, ia, fa)
 
 
 def test_every_tile_shape_and_autotune(check):
@@ -127,8 +138,9 @@ def test_every_tile_shape_and_autotune(check):
             ens.step()
         results[shape] = [ens.get_fields(i) for i in ids]
         if shape == 2:
-            rep = ens.autotune(reps=2)
-            check("autotune times every shape", sorted(rep) == [0, 1, 2], f"{rep}")
+            rep, grids = ens.autotune(reps=2)
+            check("autotune times every shape and grid mode",
+                  sorted(rep) == [0, 1, 2] and sorted(grids) == [0, 4, 16], f"{rep} {grids}")
             best = min(rep, key=rep.get)
             check("autotune keeps the fastest", ens.get_option(OPT_TILE) == best,
                   f"best={best} chosen={ens.get_option(OPT_TILE)}")
