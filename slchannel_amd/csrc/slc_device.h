@@ -141,17 +141,20 @@ __device__ inline int face_cell(const Grid& g, double z) {
 // face_cell(g, z).  z must lie in [z_lo, z_hi].
 __device__ inline int face_cell_fast(const Grid& g, double z) {
     const int nz = g.nz;
+    // the estimate is float throughout (the refinement below makes it exact)
+    const float zr = (float)z * (1.0f / (float)g.Lz), tg = (float)g.tanh_g;
+    const float rg = 1.0f / (float)g.gamma;
     float kfe;
     if (g.symmetric) {
-        float a = (float)((2.0 * z / g.Lz - 1.0) * g.tanh_g);
+        float a = (2.0f * zr - 1.0f) * tg;
         a = fminf(fmaxf(a, -0.9999999f), 0.9999999f);
         float at = 0.5f * __logf((1.0f + a) / (1.0f - a));
-        kfe = (at / (float)g.gamma + 1.0f) * (0.5f * (float)nz);
+        kfe = (at * rg + 1.0f) * (0.5f * (float)nz);
     } else {
-        float a = (float)((1.0 - z / g.Lz) * g.tanh_g);
+        float a = (1.0f - zr) * tg;
         a = fminf(fmaxf(a, -0.9999999f), 0.9999999f);
         float at = 0.5f * __logf((1.0f + a) / (1.0f - a));
-        kfe = (1.0f - at / (float)g.gamma) * (float)nz;
+        kfe = (1.0f - at * rg) * (float)nz;
     }
     int c = (int)floorf(fminf(fmaxf(kfe, 0.0f), (float)(nz - 1)));
     const double* zf = g.z_f;
@@ -286,11 +289,18 @@ struct TrajVel {
     }
 };
 
+// Trajectory velocity that is always a stored field (U* precomputed, or V^n).
+struct TrajA {
+    const double* a[3];
+    __device__ inline double at(int c, long idx) const { return a[c][idx]; }
+};
+
 // Trilinear sample of trajectory component C at (x, y, zc) where the caller
 // has already clamped z (semilag.py:512-528 with _build_iw_impl at order 2 and
 // _gather_interp's sequential (i, j, k) sum).  `c` is the face cell of zc.
-template <int C>
-__device__ inline double trilinear(const Grid& g, const TrajVel& V, double xdx, double ydy,
+// TV: TrajVel or TrajA.
+template <int C, typename TV = TrajVel>
+__device__ inline double trilinear(const Grid& g, const TV& V, double xdx, double ydy,
                                    double zc, int cell) {
     double sx = xdx + Comp<C>::shift_x;
     double ix0 = floor(sx);

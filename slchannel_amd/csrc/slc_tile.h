@@ -87,7 +87,7 @@ __device__ inline SBox s_block_box(SBox l, int* sh) {
 // Trajectory samples of the three U* components at one point (they share
 // x/dx, y/dy and the face cell of z; semilag.py:512-528 recomputes them with
 // identical results).
-__device__ inline void sample3(const Grid& g, const TrajVel& V, double x, double y, double z,
+__device__ inline void sample3(const Grid& g, const TrajA& V, double x, double y, double z,
                                double& us, double& vs, double& ws) {
     const double zc = clampd(z, g.z_lo, g.z_hi);
     const double xdx = x / g.dx, ydy = y / g.dy;
@@ -99,7 +99,7 @@ __device__ inline void sample3(const Grid& g, const TrajVel& V, double x, double
 
 // Arrival-point trajectory sample with precomputed x/dx, y/dy and face cell
 // (the same values sample3 computes; see ArrivalTabs).
-__device__ inline void sample3_at(const Grid& g, const TrajVel& V, double xdx, double ydy,
+__device__ inline void sample3_at(const Grid& g, const TrajA& V, double xdx, double ydy,
                                   double z, int cell, double& us, double& vs, double& ws) {
     const double zc = clampd(z, g.z_lo, g.z_hi);
     us = trilinear<0>(g, V, xdx, ydy, zc, cell);
@@ -139,6 +139,8 @@ __device__ __forceinline__ void sl_tile_body(
     constexpr int CAP = LDS_BYTES / (int)sizeof(T);
     const int nzi = Comp<C>::NZI(g);
     const int ntk = (nzi + TK - 1) / TK, ntj = (g.ny + TJ - 1) / TJ;
+    // the step kernels' trajectory velocity is always a stored field
+    const TrajA VA{{V.a[0], V.a[1], V.a[2]}};
     const int tk = (int)(tile % ntk);
     const long r0 = tile / ntk;
     const int tj = (int)(r0 % ntj), ti = (int)(r0 / ntj);
@@ -171,13 +173,13 @@ __device__ __forceinline__ void sl_tile_body(
                 // iterated midpoint (semilag.py:534-552); the first sample is
                 // at the arrival point for every foot depth
                 double u0, v0, w0;
-                sample3_at(g, V, A.xdx[I - 1], A.ydy[J - 1], za, A.cell[K - 1], u0, v0, w0);
+                sample3_at(g, VA, A.xdx[I - 1], A.ydy[J - 1], za, A.cell[K - 1], u0, v0, w0);
                 const double hf = (MODE == MODE_FAR) ? 0.5 * (2.0 * dt) : 0.5 * dt;
                 double xm = xa - hf * u0, ym = ya - hf * v0;
                 double zm = clampd(za - hf * w0, g.z_lo, g.z_hi);
                 for (int it = 1; it < g.n_iters; ++it) {
                     double us, vs, ws;
-                    sample3(g, V, xm, ym, zm, us, vs, ws);
+                    sample3(g, VA, xm, ym, zm, us, vs, ws);
                     xm = xa - hf * us;
                     ym = ya - hf * vs;
                     zm = clampd(za - hf * ws, g.z_lo, g.z_hi);
@@ -192,7 +194,7 @@ __device__ __forceinline__ void sl_tile_body(
                     double zn = clampd(za - hn * w0, g.z_lo, g.z_hi);
                     for (int it = 1; it < g.n_iters; ++it) {
                         double us, vs, ws;
-                        sample3(g, V, xn, yn, zn, us, vs, ws);
+                        sample3(g, VA, xn, yn, zn, us, vs, ws);
                         xn = xa - hn * us;
                         yn = ya - hn * vs;
                         zn = clampd(za - hn * ws, g.z_lo, g.z_hi);
