@@ -99,13 +99,9 @@ __device__ inline int wrap_idx(int i, int n) {
     return r < 0 ? r + n : r;
 }
 
-// floor(s) as an int; |s| is clamped to 2^30 first so the conversion is
-// defined for any input (identical to (long)floor(s) for real feet).
-__device__ inline int floor_int(double f) {
-    const double lim = 1073741824.0;
-    f = f < -lim ? -lim : (f > lim ? lim : f);
-    return (int)f;
-}
+// An integral-valued double (a floor) as an int.  The conversion instruction
+// saturates out-of-range values (feet are CFL-bounded, so it never does).
+__device__ inline int floor_int(double f) { return __double2int_rz(f); }
 
 // Continuous face index kf (semilag.py:400-411).
 __device__ inline double face_coord(const Grid& g, double z) {
@@ -242,7 +238,9 @@ __device__ inline void z_weights(const double* zn, const double* dinv, double z,
 }
 
 // Linear address of node (ib, jb, k) of component C inside its ghost-shaped array.
-// (32-bit: plans keep every field below 2^31 elements)
+// (32-bit: plans keep every field below 2^31 elements.  Signed on purpose:
+// with no wrap-around the compiler proves base+k, base+k+1 adjacent and
+// merges their loads into one 16-byte load.)
 template <int C>
 __device__ inline int node_addr(const Grid& g, int ib, int jb, int k) {
     return ((ib + Comp<C>::ox) * (int)Comp<C>::SY(g) + (jb + Comp<C>::oy)) *
@@ -457,17 +455,20 @@ __device__ inline void stencil_base(const Grid& g, double x, double y, double z,
 
 // Explicit xy-diffusion nu*(d2x + d2y) at ghost index (I, J, K)
 // (operators.py:293-367; periodic extension for u in x and v in y).
+// rdx2, rdy2: 1/dx^2, 1/dy^2 (a multiply in place of the reference's
+// division: this term carries the predictor's 1e-13 tolerance, not the
+// bit-exact feet)
 template <int C>
 __device__ inline double rhs_xy(const Grid& g, const double* __restrict__ F, int I, int J, int K,
-                                double dx2, double dy2) {
+                                double rdx2, double rdy2) {
     const long SY = Comp<C>::SY(g), SZ = Comp<C>::SZ(g);
     int Ip = I + 1, Jp = J + 1;
     if (C == 0 && Ip == g.nx + 1) Ip = 1;
     if (C == 1 && Jp == g.ny + 1) Jp = 1;
     auto at = [&](int i, int j) { return F[((long)i * SY + j) * SZ + K]; };
     double c = at(I, J);
-    double d2x = ((at(Ip, J) - 2.0 * c) + at(I - 1, J)) / dx2;
-    double d2y = ((at(I, Jp) - 2.0 * c) + at(I, J - 1)) / dy2;
+    double d2x = ((at(Ip, J) - 2.0 * c) + at(I - 1, J)) * rdx2;
+    double d2y = ((at(I, Jp) - 2.0 * c) + at(I, J - 1)) * rdy2;
     return g.nu * (d2x + d2y);
 }
 

@@ -370,7 +370,8 @@ static void launch_bdf1(const Grid& g, const TrajVel& V, const double* F, double
 
 hipError_t run_bdf1(const Grid& g, int comp, const TrajVel& V, const double* F, double dt,
                     double* out, unsigned long long* ncl, hipStream_t s) {
-    double dx2 = g.dx * g.dx, dy2 = g.dy * g.dy;
+    // reciprocal squares (rhs_xy multiplies)
+    double dx2 = 1.0 / (g.dx * g.dx), dy2 = 1.0 / (g.dy * g.dy);
     SLC_DISPATCH_C(comp, launch_bdf1, g, V, F, dt, dx2, dy2, out, ncl, s);
     return hipGetLastError();
 }
@@ -402,7 +403,8 @@ static void launch_bdf2(const Grid& g, int pass, const TrajVel& V, const double*
 hipError_t run_bdf2(const Grid& g, int comp, int pass, const TrajVel& V, const double* Fn,
                     const double* Fm, double dt, double* out, unsigned long long* ncl,
                     hipStream_t s) {
-    double dx2 = g.dx * g.dx, dy2 = g.dy * g.dy;
+    // reciprocal squares (rhs_xy multiplies)
+    double dx2 = 1.0 / (g.dx * g.dx), dy2 = 1.0 / (g.dy * g.dy);
     double dt_eff = (2.0 / 3.0) * dt;
     SLC_DISPATCH_C(comp, launch_bdf2, g, pass, V, Fn, Fm, dt, dt_eff, dx2, dy2, out, ncl, s);
     return hipGetLastError();

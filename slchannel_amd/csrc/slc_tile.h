@@ -245,7 +245,7 @@ __device__ __forceinline__ void sl_tile_body(
     // ---- phase 2: contract -------------------------------------------------
     const double* zn = znodes<C>(g);
     const double* dinv = Comp<C>::centers ? g.dinv_cP : g.dinv_fP;
-    const double dx2 = g.dx * g.dx, dy2 = g.dy * g.dy;
+    const double rdx2 = 1.0 / (g.dx * g.dx), rdy2 = 1.0 / (g.dy * g.dy);
 #pragma unroll
     for (int p = 0; p < PT; ++p) {
         if (idx[p] < 0) continue;
@@ -299,11 +299,11 @@ __device__ __forceinline__ void sl_tile_body(
         } else if (MODE == MODE_ADVECT || MODE == MODE_FAR) {
             out[o] = acc;
         } else if (MODE == MODE_BDF1) {
-            out[o] = acc + dt * rhs_xy<C>(g, F, I, J, K, dx2, dy2);
+            out[o] = acc + dt * rhs_xy<C>(g, F, I, J, K, rdx2, rdy2);
         } else {
             const double far = out[o];
-            const double rn = rhs_xy<C>(g, F, I, J, K, dx2, dy2);
-            const double rm = rhs_xy<C>(g, Fm, I, J, K, dx2, dy2);
+            const double rn = rhs_xy<C>(g, F, I, J, K, rdx2, rdy2);
+            const double rm = rhs_xy<C>(g, Fm, I, J, K, rdx2, rdy2);
             const double rh = 2.0 * rn - rm;
             const double dt_eff = (2.0 / 3.0) * dt;
             const double pred = fma(far, -1.0 / 3.0, acc * (4.0 / 3.0));
