@@ -216,11 +216,12 @@ __device__ __forceinline__ void sl_tile_body(
     const T* __restrict__ S = (MODE == MODE_FAR) ? Sm : Sn;
     const SBox b = s_block_box(l, sh);
     const int BX = b.xmax - b.xmin + P, BY = b.ymax - b.ymin + P, BZ = b.kmax - b.kmin + P;
+    const int BZS = BZ;  // row stride of the staged box (an odd stride measured no faster)
     // LDS: the box (z fastest), then one int per (x, y) row: its global base
     const int rows = BX * BY;
-    const int nbox = rows * BZ;
+    const int nbox = rows * BZS;
     const bool fits = b.xmin <= b.xmax &&
-                      (long)BX * BY * BZ * (long)sizeof(T) + (long)BX * BY * 4 <= LDS_BYTES;
+                      (long)BX * BY * BZS * (long)sizeof(T) + (long)BX * BY * 4 <= LDS_BYTES;
     if (fits) {
         int* rowbase = reinterpret_cast<int*>(lds + nbox);
         for (int r = threadIdx.x; r < rows; r += kSB) {
@@ -233,11 +234,12 @@ __device__ __forceinline__ void sl_tile_body(
         // comes from a float reciprocal, made exact by one integer
         // correction each way (the estimate is within 1 of the quotient)
         const float rbz = 1.0f / (float)BZ;
-        for (int e = threadIdx.x; e < nbox; e += kSB) {
+        for (int e = threadIdx.x; e < rows * BZ; e += kSB) {
             int row = (int)(((float)e + 0.5f) * rbz);
             row += (row + 1) * BZ <= e ? 1 : 0;
             row -= row * BZ > e ? 1 : 0;
-            lds[e] = S[rowbase[row] + (e - row * BZ)];
+            const int zz = e - row * BZ;
+            lds[row * BZS + zz] = S[rowbase[row] + zz];
         }
     }
     __syncthreads();
@@ -263,7 +265,7 @@ __device__ __forceinline__ void sl_tile_body(
                 double ai = 0.0;
 #pragma unroll
                 for (int j = 0; j < P; ++j) {
-                    const T* col = lds + ((ox + i) * BY + (oy + j)) * BZ + oz;
+                    const T* col = lds + ((ox + i) * BY + (oy + j)) * BZS + oz;
                     double sj = 0.0;
 #pragma unroll
                     for (int k = 0; k < P; ++k) sj = fma(wz[k], (double)col[k], sj);
