@@ -121,6 +121,118 @@ __device__ inline void make_st(const Grid& g, double x, double y, double z, Foot
     stencil_base<C, P>(g, x, y, f.z, f.bx, f.by, f.bk, f.tx, f.ty);
 }
 
+// Stages the stencil bounding box `b` of a tile's feet (field S) into `lds`,
+// using at most cap_bytes: the box (z fastest), then one int per (x, y) row,
+// its global base.  Returns false (nothing staged) when the box does not fit.
+// Ends with a barrier.
+template <typename T, int C, int P>
+__device__ inline bool stage_box(const Grid& g, const T* __restrict__ S, const SBox& b,
+                                 T* __restrict__ lds, long cap_bytes, int& BY, int& BZ) {
+    const int BX = b.xmax - b.xmin + P;
+    BY = b.ymax - b.ymin + P;
+    BZ = b.kmax - b.kmin + P;
+    const int rows = BX * BY;
+    const int nbox = rows * BZ;
+    const bool fits = b.xmin <= b.xmax &&
+                      (long)nbox * (long)sizeof(T) + (long)rows * 4 <= cap_bytes;
+    if (fits) {
+        int* rowbase = reinterpret_cast<int*>(lds + nbox);
+        for (int r = threadIdx.x; r < rows; r += kSB) {
+            const int x = r / BY, y = r - (r / BY) * BY;
+            rowbase[r] = node_addr<C>(g, wrap_idx(b.xmin + x, g.nx),
+                                      wrap_idx(b.ymin + y, g.ny), b.kmin);
+        }
+        __syncthreads();
+        // flat element loop (z fastest: coalesced rows); the quotient e/BZ
+        // comes from a float reciprocal, made exact by one integer
+        // correction each way (the estimate is within 1 of the quotient)
+        const float rbz = 1.0f / (float)BZ;
+        for (int e = threadIdx.x; e < nbox; e += kSB) {
+            int row = (int)(((float)e + 0.5f) * rbz);
+            row += (row + 1) * BZ <= e ? 1 : 0;
+            row -= row * BZ > e ? 1 : 0;
+            const int zz = e - row * BZ;
+            lds[e] = S[rowbase[row] + zz];
+        }
+    }
+    __syncthreads();
+    return fits;
+}
+
+// Order-P gather of one foot: Lagrange weights, then the staged z -> y -> x
+// contraction with FMAs, from the LDS box when it was staged, else from S.
+template <typename T, int C, int P>
+__device__ inline double contract(const Grid& g, const FootSt& f, const SBox& b, bool fits,
+                                  const T* __restrict__ lds, int BY, int BZ,
+                                  const T* __restrict__ S) {
+    const double* zn = znodes<C>(g);
+    const double* dinv = Comp<C>::centers ? g.dinv_cP : g.dinv_fP;
+    double wx[P], wy[P], wz[P];
+    uniform_weights<P>(f.tx, wx);
+    uniform_weights<P>(f.ty, wy);
+    z_weights<P>(zn, dinv, f.z, f.bk, wz);
+    double acc = 0.0;
+    if (fits) {
+        const int ox = f.bx - b.xmin, oy = f.by - b.ymin, oz = f.bk - b.kmin;
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            double ai = 0.0;
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const T* col = lds + ((ox + i) * BY + (oy + j)) * BZ + oz;
+                double sj = 0.0;
+#pragma unroll
+                for (int k = 0; k < P; ++k) sj = fma(wz[k], (double)col[k], sj);
+                ai = fma(wy[j], sj, ai);
+            }
+            acc = fma(wx[i], ai, acc);
+        }
+    } else {
+#pragma unroll 1
+        for (int i = 0; i < P; ++i) {
+            const int gx = wrap_idx(f.bx + i, g.nx);
+            double ai = 0.0;
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const int gy = wrap_idx(f.by + j, g.ny);
+                const T* col = S + node_addr<C>(g, gx, gy, f.bk);
+                double sj = 0.0;
+#pragma unroll
+                for (int k = 0; k < P; ++k) sj = fma(wz[k], (double)col[k], sj);
+                ai = fma(wy[j], sj, ai);
+            }
+            acc = fma(uniform_weight_m<P>(f.tx, i), ai, acc);
+        }
+    }
+    return acc;
+}
+
+__device__ inline void box_add(SBox& l, const FootSt& f) {
+    l.xmin = min(l.xmin, f.bx); l.xmax = max(l.xmax, f.bx);
+    l.ymin = min(l.ymin, f.by); l.ymax = max(l.ymax, f.by);
+    l.kmin = min(l.kmin, f.bk); l.kmax = max(l.kmax, f.bk);
+}
+
+// Iterated midpoint (semilag.py:534-552) from the arrival point with its
+// shared first sample (u0, v0, w0); returns the unclamped foot.
+__device__ inline void trace_foot(const Grid& g, const TrajA& V, double xa, double ya,
+                                  double za, double u0, double v0, double w0, double h,
+                                  double& x, double& y, double& z) {
+    double xm = xa - h * u0, ym = ya - h * v0;
+    double zm = clampd(za - h * w0, g.z_lo, g.z_hi);
+    for (int it = 1; it < g.n_iters; ++it) {
+        double us, vs, ws;
+        sample3(g, V, xm, ym, zm, us, vs, ws);
+        xm = xa - h * us;
+        ym = ya - h * vs;
+        zm = clampd(za - h * ws, g.z_lo, g.z_hi);
+    }
+    if (g.n_iters == 0) { xm = xa; ym = ya; zm = za; }
+    x = 2.0 * xm - xa;
+    y = 2.0 * ym - ya;
+    z = 2.0 * zm - za;
+}
+
 // T: storage type of the gathered field (double, or float for the FP32-storage
 // variant: values are staged and read as float, weights and accumulation stay
 // FP64).  F / Fm are always the FP64 fields (explicit xy-diffusion);
@@ -136,7 +248,6 @@ __device__ __forceinline__ void sl_tile_body(
     constexpr int NPT = TI * TJ * TK;
     static_assert(NPT % kSB == 0, "tile must be a multiple of the block");
     constexpr int PT = NPT / kSB;
-    constexpr int CAP = LDS_BYTES / (int)sizeof(T);
     const int nzi = Comp<C>::NZI(g);
     const int ntk = (nzi + TK - 1) / TK, ntj = (g.ny + TJ - 1) / TJ;
     // the step kernels' trajectory velocity is always a stored field
@@ -170,126 +281,37 @@ __device__ __forceinline__ void sl_tile_body(
                 y = ft.y[n];
                 z = ft.z[n];
             } else {
-                // iterated midpoint (semilag.py:534-552); the first sample is
-                // at the arrival point for every foot depth
+                // the first sample is at the arrival point for every foot depth
                 double u0, v0, w0;
                 sample3_at(g, VA, A.xdx[I - 1], A.ydy[J - 1], za, A.cell[K - 1], u0, v0, w0);
-                const double hf = (MODE == MODE_FAR) ? 0.5 * (2.0 * dt) : 0.5 * dt;
-                double xm = xa - hf * u0, ym = ya - hf * v0;
-                double zm = clampd(za - hf * w0, g.z_lo, g.z_hi);
-                for (int it = 1; it < g.n_iters; ++it) {
-                    double us, vs, ws;
-                    sample3(g, VA, xm, ym, zm, us, vs, ws);
-                    xm = xa - hf * us;
-                    ym = ya - hf * vs;
-                    zm = clampd(za - hf * ws, g.z_lo, g.z_hi);
-                }
-                if (g.n_iters == 0) { xm = xa; ym = ya; zm = za; }
-                x = 2.0 * xm - xa;
-                y = 2.0 * ym - ya;
-                z = 2.0 * zm - za;
-                if (MODE == MODE_FAR) {
-                    const double hn = 0.5 * dt;
-                    double xn = xa - hn * u0, yn = ya - hn * v0;
-                    double zn = clampd(za - hn * w0, g.z_lo, g.z_hi);
-                    for (int it = 1; it < g.n_iters; ++it) {
-                        double us, vs, ws;
-                        sample3(g, VA, xn, yn, zn, us, vs, ws);
-                        xn = xa - hn * us;
-                        yn = ya - hn * vs;
-                        zn = clampd(za - hn * ws, g.z_lo, g.z_hi);
-                    }
-                    if (g.n_iters == 0) { xn = xa; yn = ya; zn = za; }
-                    ft.x[n] = 2.0 * xn - xa;
-                    ft.y[n] = 2.0 * yn - ya;
-                    ft.z[n] = 2.0 * zn - za;
+                trace_foot(g, VA, xa, ya, za, u0, v0, w0,
+                           MODE == MODE_FAR ? 0.5 * (2.0 * dt) : 0.5 * dt, x, y, z);
+                if (MODE == MODE_FAR) {  // the near foot, for MODE_NEAR
+                    double xn, yn, zn;
+                    trace_foot(g, VA, xa, ya, za, u0, v0, w0, 0.5 * dt, xn, yn, zn);
+                    ft.x[n] = xn;
+                    ft.y[n] = yn;
+                    ft.z[n] = zn;
                 }
             }
             make_st<C, P>(g, x, y, z, st[p], ncl);
-            l.xmin = min(l.xmin, st[p].bx); l.xmax = max(l.xmax, st[p].bx);
-            l.ymin = min(l.ymin, st[p].by); l.ymax = max(l.ymax, st[p].by);
-            l.kmin = min(l.kmin, st[p].bk); l.kmax = max(l.kmax, st[p].bk);
+            box_add(l, st[p]);
         }
     }
 
     // ---- stage the tile's stencil box in LDS -------------------------------
     const T* __restrict__ S = (MODE == MODE_FAR) ? Sm : Sn;
     const SBox b = s_block_box(l, sh);
-    const int BX = b.xmax - b.xmin + P, BY = b.ymax - b.ymin + P, BZ = b.kmax - b.kmin + P;
-    const int BZS = BZ;  // row stride of the staged box (an odd stride measured no faster)
-    // LDS: the box (z fastest), then one int per (x, y) row: its global base
-    const int rows = BX * BY;
-    const int nbox = rows * BZS;
-    const bool fits = b.xmin <= b.xmax &&
-                      (long)BX * BY * BZS * (long)sizeof(T) + (long)BX * BY * 4 <= LDS_BYTES;
-    if (fits) {
-        int* rowbase = reinterpret_cast<int*>(lds + nbox);
-        for (int r = threadIdx.x; r < rows; r += kSB) {
-            const int x = r / BY, y = r - (r / BY) * BY;
-            rowbase[r] = node_addr<C>(g, wrap_idx(b.xmin + x, g.nx),
-                                      wrap_idx(b.ymin + y, g.ny), b.kmin);
-        }
-        __syncthreads();
-        // flat element loop (z fastest: coalesced rows); the quotient e/BZ
-        // comes from a float reciprocal, made exact by one integer
-        // correction each way (the estimate is within 1 of the quotient)
-        const float rbz = 1.0f / (float)BZ;
-        for (int e = threadIdx.x; e < rows * BZ; e += kSB) {
-            int row = (int)(((float)e + 0.5f) * rbz);
-            row += (row + 1) * BZ <= e ? 1 : 0;
-            row -= row * BZ > e ? 1 : 0;
-            const int zz = e - row * BZ;
-            lds[row * BZS + zz] = S[rowbase[row] + zz];
-        }
-    }
-    __syncthreads();
+    int BY = 0, BZ = 0;
+    const bool fits = stage_box<T, C, P>(g, S, b, lds, LDS_BYTES, BY, BZ);
 
     // ---- phase 2: contract -------------------------------------------------
-    const double* zn = znodes<C>(g);
-    const double* dinv = Comp<C>::centers ? g.dinv_cP : g.dinv_fP;
     const double rdx2 = 1.0 / (g.dx * g.dx), rdy2 = 1.0 / (g.dy * g.dy);
 #pragma unroll
     for (int p = 0; p < PT; ++p) {
         if (idx[p] < 0) continue;
         const long n = idx[p];
-        const FootSt& f = st[p];
-        double wx[P], wy[P], wz[P];
-        uniform_weights<P>(f.tx, wx);
-        uniform_weights<P>(f.ty, wy);
-        z_weights<P>(zn, dinv, f.z, f.bk, wz);
-        double acc = 0.0;
-        if (fits) {
-            const int ox = f.bx - b.xmin, oy = f.by - b.ymin, oz = f.bk - b.kmin;
-#pragma unroll
-            for (int i = 0; i < P; ++i) {
-                double ai = 0.0;
-#pragma unroll
-                for (int j = 0; j < P; ++j) {
-                    const T* col = lds + ((ox + i) * BY + (oy + j)) * BZS + oz;
-                    double sj = 0.0;
-#pragma unroll
-                    for (int k = 0; k < P; ++k) sj = fma(wz[k], (double)col[k], sj);
-                    ai = fma(wy[j], sj, ai);
-                }
-                acc = fma(wx[i], ai, acc);
-            }
-        } else {
-#pragma unroll 1
-            for (int i = 0; i < P; ++i) {
-                const int gx = wrap_idx(f.bx + i, g.nx);
-                double ai = 0.0;
-#pragma unroll
-                for (int j = 0; j < P; ++j) {
-                    const int gy = wrap_idx(f.by + j, g.ny);
-                    const T* col = S + node_addr<C>(g, gx, gy, f.bk);
-                    double sj = 0.0;
-#pragma unroll
-                    for (int k = 0; k < P; ++k) sj = fma(wz[k], (double)col[k], sj);
-                    ai = fma(wy[j], sj, ai);
-                }
-                acc = fma(uniform_weight_m<P>(f.tx, i), ai, acc);
-            }
-        }
+        const double acc = contract<T, C, P>(g, st[p], b, fits, lds, BY, BZ, S);
         // arrival ghost index of n
         const unsigned un = (unsigned)n;
         const int K = (int)(un % (unsigned)nzi) + 1;
