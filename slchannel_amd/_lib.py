@@ -1,5 +1,5 @@
-"""ctypes binding of libslc_hip.so (the C ABI declared in include/slc.h and
-include/slc_ens.h).
+"""ctypes binding of libslc_hip.so (the C ABI declared in include/slc.h,
+include/slc_ens.h and include/slc_stats.h).
 
 The library is loaded from the package directory only; if it is missing the
 import of any GPU entry point raises -- there is no CPU fallback.
@@ -12,8 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # SLCHANNEL_LIB overrides the library file (e.g. a build with another tile shape)
 LIB_PATH = os.environ.get("SLCHANNEL_LIB") or os.path.join(HERE, "libslc_hip.so")
 
-# Every symbol include/slc.h and include/slc_ens.h declare (checked by the
-# CPU test suite).
+# Every symbol include/slc.h, include/slc_ens.h and include/slc_stats.h
+# declare (checked by the CPU test suite).
 EXPORTS = (
     "slc_version", "slc_last_error", "slc_plan_create", "slc_plan_destroy",
     "slc_plan_workspace_bytes", "slc_apply_bc", "slc_departure", "slc_gather",
@@ -25,6 +25,8 @@ EXPORTS = (
     "slc_ens_remove", "slc_ens_count", "slc_ens_bytes", "slc_ens_set_fields",
     "slc_ens_get_fields", "slc_ens_stats", "slc_ens_step", "slc_ens_set_option",
     "slc_ens_get_option", "slc_ens_autotune",
+    "slc_stats_create", "slc_stats_destroy", "slc_stats_workspace_bytes",
+    "slc_stats_accumulate",
 )
 
 
@@ -96,6 +98,10 @@ def load():
         "slc_ens_set_option": ([vp, i, i], i),
         "slc_ens_get_option": ([vp, i, ctypes.POINTER(i)], i),
         "slc_ens_autotune": ([vp, i, vp, ctypes.c_char_p, ctypes.c_int64], i),
+        "slc_stats_create": ([i, i, i, i, i, vp, vp, ctypes.POINTER(vp)], i),
+        "slc_stats_destroy": ([vp], i),
+        "slc_stats_workspace_bytes": ([vp, ctypes.POINTER(ctypes.c_int64)], i),
+        "slc_stats_accumulate": ([vp, vp, vp, vp, vp, vp, vp], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -1,7 +1,7 @@
 """CPU-only checks of the native boundary and the host logic (no GPU calls).
 
-* libslc_hip.so loads and exports every function include/slc.h and
-  include/slc_ens.h declare;
+* libslc_hip.so loads and exports every function include/slc.h,
+  include/slc_ens.h and include/slc_stats.h declare;
 * the ctypes table in slchannel_amd/_lib.py binds exactly those symbols;
 * the drop-in classes reject bad arguments with the reference's errors
   (slchannel/semilag.py:204-219) before touching a device;
@@ -20,7 +20,7 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     out = set()
-    for h in ("slc.h", "slc_ens.h"):
+    for h in ("slc.h", "slc_ens.h", "slc_stats.h"):
         hdr = open(os.path.join(REPO, "include", h)).read()
         out |= set(re.findall(r"^(?:int|const char\*)\s+(slc_\w+)\(", hdr, re.M))
     return out
