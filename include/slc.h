@@ -170,6 +170,17 @@ int slc_project_step(slc_plan* plan, double* u, double* v, double* w, double rhs
  * (slchannel/utils.py:147-163). */
 int slc_bulk_velocity(slc_plan* plan, const double* u, double* out, void* stream);
 
+/* Run-loop diagnostics of the current fields, written to out (5 device
+ * doubles, no host synchronisation): [0] max|div| (compute_divergence,
+ * utils.py:166-180), [1] u_tau (compute_u_tau, utils.py:116-146), [2] the
+ * advective CFL dti * dt and [3] dti (compute_cfl_fused,
+ * operators.py:839-940), [4] the number of NaN/Inf entries of u, v, w and p
+ * (the run loop's guard, solver.py:1436-1479; p may be null).  Replaces the
+ * per-n_out diagnostics of the run loop (solver.py:1355-1392); the maxima
+ * propagate NaN. */
+int slc_diagnostics(slc_plan* plan, const double* u, const double* v, const double* w,
+                    const double* p, double dt, double* out, void* stream);
+
 /* Exact bulk-flux constraint: u += U_bulk - ub(u) on the interior, ghosts of
  * u refreshed.  out (4 device doubles): [0] correction, [1] forcing =
  * correction/dt, [2] bulk velocity after the shift, [3] before it.

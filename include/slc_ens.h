@@ -68,6 +68,12 @@ int slc_ens_get_fields(slc_ens* ens, int id, double* u, double* v, double* w, do
  * u_bulk before the shift. */
 int slc_ens_stats(slc_ens* ens, int id, double* out, void* stream);
 
+/* Run-loop diagnostics of the member's current fields (slc_diagnostics:
+ * max|div|, u_tau, CFL at the member's dt, dti, non-finite entries) into out
+ * (5 device doubles), ordered on `stream` like the other calls: the serving
+ * loop's NaN/blow-up guard without a host synchronisation. */
+int slc_ens_diagnostics(slc_ens* ens, int id, double* out, void* stream);
+
 /* One SL-BDF2 step of every member. */
 int slc_ens_step(slc_ens* ens, void* stream);
 

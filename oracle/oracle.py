@@ -543,6 +543,58 @@ def compute_bulk_velocity(u, cell_vol_ratio, total_volume):
 
 
 # ---------------------------------------------------------------------------
+# Run-loop diagnostics
+# ---------------------------------------------------------------------------
+
+
+def compute_u_tau(u, z_c, nu, top_wall_bc_type="dirichlet"):
+    """utils.py:116-146: plane means of u[:, :, 1] and u[:, :, -2] over the
+    whole ghost-shaped x-y extent."""
+    dist = z_c[1]
+    u_tau_bot = np.sqrt(np.abs(nu * np.mean(u[:, :, 1]) / dist))
+    if top_wall_bc_type == "neumann":
+        return u_tau_bot
+    u_tau_top = np.sqrt(np.abs(nu * np.mean(u[:, :, -2]) / dist))
+    return 0.5 * (u_tau_bot + u_tau_top)
+
+
+def compute_cfl_fused(u, v, w, nx, ny, nz, dx, dy, dz_f, dz_c):
+    """operators.py:839-940: max over cells of the x, y and z inverse
+    time-scales (4-point averages of the staggered components)."""
+    dxi, dyi = 1.0 / dx, 1.0 / dy
+    dzfi = (1.0 / np.asarray(dz_f))[0:nz].reshape(1, 1, -1)
+    dzci = (1.0 / np.asarray(dz_c))[1:nz].reshape(1, 1, -1)
+    I, J, K = slice(1, nx + 1), slice(1, ny + 1), slice(1, nz + 1)
+    ux = np.abs(u[I, J, K])
+    vx = 0.25 * np.abs(v[I, J, K] + v[I, 0:ny, K] + v[2:nx + 2, J, K] + v[2:nx + 2, 0:ny, K])
+    wx = 0.25 * np.abs(w[I, J, K] + w[I, J, 0:nz] + w[2:nx + 2, J, K] + w[2:nx + 2, J, 0:nz])
+    dtix = ux * dxi + vx * dyi + wx * dzfi
+    uy = 0.25 * np.abs(u[I, J, K] + u[I, 2:ny + 2, K] + u[0:nx, 2:ny + 2, K] + u[0:nx, J, K])
+    vy = np.abs(v[I, J, K])
+    wy = 0.25 * np.abs(w[I, J, K] + w[I, 2:ny + 2, K] + w[I, 2:ny + 2, 0:nz] + w[I, J, 0:nz])
+    dtiy = uy * dxi + vy * dyi + wy * dzfi
+    Kz, Kz1 = slice(1, nz), slice(2, nz + 1)
+    uz = 0.25 * np.abs(u[I, J, Kz] + u[0:nx, J, Kz] + u[0:nx, J, Kz1] + u[I, J, Kz1])
+    vz = 0.25 * np.abs(v[I, J, Kz] + v[I, 0:ny, Kz] + v[I, 0:ny, Kz1] + v[I, J, Kz1])
+    wz = np.abs(w[I, J, Kz])
+    dtiz = uz * dxi + vz * dyi + wz * dzci
+    parts = [dtix.max(), dtiy.max()] + ([dtiz.max()] if nz > 1 else [])
+    return float(np.max(parts))
+
+
+def run_diagnostics(u, v, w, p, nx, ny, nz, dx, dy, dz_f, dz_c, z_c, nu, dt,
+                    top_wall_bc_type="dirichlet"):
+    """The run loop's per-n_out diagnostics and NaN/Inf guard
+    (solver.py:1355-1392, 1436-1479): [max|div|, u_tau, dti*dt, dti,
+    non-finite entries of u, v, w, p]."""
+    div = compute_divergence(u, v, w, nx, ny, nz, dx, dy, dz_f)
+    dti = compute_cfl_fused(u, v, w, nx, ny, nz, dx, dy, dz_f, dz_c)
+    bad = sum(int(np.count_nonzero(~np.isfinite(a))) for a in (u, v, w, p) if a is not None)
+    return np.array([np.max(np.abs(div)), compute_u_tau(u, z_c, nu, top_wall_bc_type),
+                     dti * dt, dti, float(bad)])
+
+
+# ---------------------------------------------------------------------------
 # The step
 # ---------------------------------------------------------------------------
 

@@ -19,12 +19,12 @@ EXPORTS = (
     "slc_plan_workspace_bytes", "slc_apply_bc", "slc_departure", "slc_gather",
     "slc_stencil", "slc_locate_z", "slc_advect", "slc_bdf1_predict", "slc_bdf2_predict",
     "slc_zsolve", "slc_divergence", "slc_poisson", "slc_project", "slc_project_step",
-    "slc_bulk_velocity", "slc_bulk_force", "slc_set_option", "slc_profile_enable",
-    "slc_profile_report",
+    "slc_bulk_velocity", "slc_bulk_force", "slc_diagnostics", "slc_set_option",
+    "slc_profile_enable", "slc_profile_report",
     "slc_ens_create", "slc_ens_destroy", "slc_ens_member_bytes", "slc_ens_add",
     "slc_ens_remove", "slc_ens_count", "slc_ens_bytes", "slc_ens_set_fields",
     "slc_ens_get_fields", "slc_ens_stats", "slc_ens_step", "slc_ens_set_option",
-    "slc_ens_get_option", "slc_ens_autotune",
+    "slc_ens_get_option", "slc_ens_autotune", "slc_ens_diagnostics",
     "slc_stats_create", "slc_stats_destroy", "slc_stats_workspace_bytes",
     "slc_stats_accumulate",
 )
@@ -81,6 +81,7 @@ def load():
         "slc_project_step": ([vp, vp, vp, vp, d, d, vp, vp], i),
         "slc_bulk_velocity": ([vp, vp, vp, vp], i),
         "slc_bulk_force": ([vp, vp, vp, vp, d, d, vp, vp], i),
+        "slc_diagnostics": ([vp, vp, vp, vp, vp, d, vp, vp], i),
         "slc_set_option": ([vp, i, i], i),
         "slc_profile_enable": ([vp, i], i),
         "slc_profile_report": ([vp, ctypes.c_char_p, ctypes.c_int64], i),
@@ -98,6 +99,7 @@ def load():
         "slc_ens_set_option": ([vp, i, i], i),
         "slc_ens_get_option": ([vp, i, ctypes.POINTER(i)], i),
         "slc_ens_autotune": ([vp, i, vp, ctypes.c_char_p, ctypes.c_int64], i),
+        "slc_ens_diagnostics": ([vp, i, vp, vp], i),
         "slc_stats_create": ([i, i, i, i, i, vp, vp, ctypes.POINTER(vp)], i),
         "slc_stats_destroy": ([vp], i),
         "slc_stats_workspace_bytes": ([vp, ctypes.POINTER(ctypes.c_int64)], i),

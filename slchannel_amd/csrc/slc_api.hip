@@ -314,6 +314,8 @@ int slc::plan_create(const slc_grid_desc* d, int device, bool spectral, slc_plan
     p->max_partials = bulk_partials_needed(g);
     if ((rc = dev_alloc(p, sizeof(double) * p->max_partials, &ptr))) { slc_plan_destroy(p); return rc; }
     p->partials = (double*)ptr;
+    if ((rc = dev_alloc(p, sizeof(double) * 5 * kDiagMaxBlocks, &ptr))) { slc_plan_destroy(p); return rc; }
+    p->diag_part = (double*)ptr;
 
     if (!spectral) {
         *out = p;

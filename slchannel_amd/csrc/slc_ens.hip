@@ -508,9 +508,10 @@ int64_t member_bytes(int nx, int ny, int nz, int order, int storage) {
     if (storage == 32)
         for (int c = 0; c < 3; ++c) b += 2L * 4 * field_elems(c, nx, ny, nz);
     b += spectral_bytes(nx, ny, nz);
-    // tables: z (2 P + 4 columns of nz+2), arrival, Poisson, z-solve factors, partials
+    // tables: z (2 P + 4 columns of nz+2), arrival, Poisson, z-solve factors,
+    // bulk and diagnostics partials
     b += 8L * (nz + 2) * (2 * order + 8 + 18) + 8L * (nx + ny) * 4 + 4L * 3 * nz +
-         8L * bulk_blocks(nx, ny, nz) + 4096;
+         8L * bulk_blocks(nx, ny, nz) + 8L * 5 * kDiagMaxBlocks + 4096;
     return b;
 }
 
@@ -758,6 +759,20 @@ int slc_ens_stats(slc_ens* e, int id, double* out, void* stream) {
     DeviceGuard dg(e->device);
     if (int rc = enter(e, stream)) return rc;
     HIP_OK(hipMemcpyAsync(out, m->stats, 8 * ES_COUNT, hipMemcpyDeviceToDevice, e->es));
+    return leave(e, stream);
+}
+
+int slc_ens_diagnostics(slc_ens* e, int id, double* out, void* stream) {
+    if (int rc = ens_check(e)) return rc;
+    Member* m = find(e, id);
+    if (!m) return fail(SLC_EINVAL, "no member with id %d", id);
+    if (!out) return fail(SLC_EINVAL, "null argument");
+    DeviceGuard dg(e->device);
+    if (int rc = enter(e, stream)) return rc;
+    const int a = ens_a(m->host_state);
+    if (int rc = slc_diagnostics(m->plan, m->fld[a][0], m->fld[a][1], m->fld[a][2], m->p, m->dt,
+                                 out, e->es))
+        return rc;
     return leave(e, stream);
 }
 

@@ -67,6 +67,7 @@ struct slc_plan {
     bool zvalid[3] = {false, false, false};
     double* partials = nullptr;
     int max_partials = 0;
+    double* diag_part = nullptr;  // slc_diagnostics partials (5 per workgroup)
     double* us[3] = {nullptr, nullptr, nullptr};  // U* = 2V^n - V^{n-1} (ghost-shaped)
     double* nfoot = nullptr;                      // near feet of one component (3 x N)
     ArrivalTabs at[3] = {};
@@ -92,6 +93,10 @@ namespace slc {
 
 // hipMalloc tracked by the plan (freed by slc_plan_destroy).
 int dev_alloc(slc_plan* p, size_t bytes, void** out);
+
+// Workgroups of the diagnostics pass (slc_diagnostics); the plan holds
+// 5 doubles of partials per workgroup.
+constexpr int kDiagMaxBlocks = 2048;
 
 // Plan construction; spectral = false skips the Poisson planes, spectra and
 // rocFFT plans (members of an ensemble share per-size-class ones).

@@ -128,13 +128,19 @@ __device__ inline void make_st(const Grid& g, double x, double y, double z, Foot
 template <typename T, int C, int P>
 __device__ inline bool stage_box(const Grid& g, const T* __restrict__ S, const SBox& b,
                                  T* __restrict__ lds, long cap_bytes, int& BY, int& BZ) {
-    const int BX = b.xmax - b.xmin + P;
-    BY = b.ymax - b.ymin + P;
-    BZ = b.kmax - b.kmin + P;
+    // extents in 64 bits, each bounded before the product: feet far outside
+    // the domain (or non-finite ones, whose floors saturate) would overflow
+    // int extents and must take the direct-load path
+    const long ex = (long)b.xmax - b.xmin + P, ey = (long)b.ymax - b.ymin + P,
+               ez = (long)b.kmax - b.kmin + P;
+    const bool fits = b.xmin <= b.xmax && ex <= cap_bytes && ey <= cap_bytes &&
+                      ez <= cap_bytes &&
+                      ex * ey * ez * (long)sizeof(T) + ex * ey * 4 <= cap_bytes;
+    const int BX = fits ? (int)ex : 0;
+    BY = fits ? (int)ey : 1;
+    BZ = fits ? (int)ez : 1;
     const int rows = BX * BY;
     const int nbox = rows * BZ;
-    const bool fits = b.xmin <= b.xmax &&
-                      (long)nbox * (long)sizeof(T) + (long)rows * 4 <= cap_bytes;
     if (fits) {
         int* rowbase = reinterpret_cast<int*>(lds + nbox);
         for (int r = threadIdx.x; r < rows; r += kSB) {

@@ -124,9 +124,13 @@ __global__ __launch_bounds__(kTB) void gather_tiled_kernel(Grid g, const double*
         }
     }
     const BBox b = block_bbox(l, sh);
-    const int BX = b.xmax - b.xmin + P, BY = b.ymax - b.ymin + P, BZ = b.kmax - b.kmin + P;
+    // 64-bit extents, each bounded before the product (feet far outside the
+    // domain or non-finite ones would overflow int extents)
+    const long ex = (long)b.xmax - b.xmin + P, ey = (long)b.ymax - b.ymin + P,
+               ez = (long)b.kmax - b.kmin + P;
     const bool any = b.xmin <= b.xmax;
-    const bool fits = any && (long)BX * BY * BZ <= CAP;
+    const bool fits = any && ex <= CAP && ey <= CAP && ez <= CAP && ex * ey * ez <= CAP;
+    const int BX = fits ? (int)ex : 0, BY = fits ? (int)ey : 1, BZ = fits ? (int)ez : 1;
     if (fits) stage_box<C>(g, F, lds, b.xmin, b.ymin, b.kmin, BX, BY, BZ);
     __syncthreads();
 

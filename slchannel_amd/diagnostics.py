@@ -1,4 +1,5 @@
-"""Run diagnostics used to size dt (off the hot path; torch ops on the device).
+"""Run diagnostics: dt sizing (torch ops on the device, off the hot path)
+and the blow-up guard over the device diagnostics of slc_diagnostics.
 
 compute_cfl restates compute_cfl_fused (slchannel/operators.py:839-940): the
 inverse trajectory time-scale max over cells of (|u|/dx + |v|/dy + |w|/dz)
@@ -38,3 +39,29 @@ def cfl_dt(flow, cfl_target):
     dti = compute_cfl(flow.u, flow.v, flow.w, flow.nx, flow.ny, flow.nz, flow.dx, flow.dy,
                       flow.dz_f, flow.dz_c)
     return cfl_target / (dti if dti > 1e-10 else 1.0)
+
+
+class BlowupGuard:
+    """solver.py:1202-1213 (factor: time.blowup_u_tau_factor, default 2,
+    solver.py:234): True once u_tau has exceeded factor x nominal on
+    three consecutive diagnostics (a single spike resets).  Fed from rows of
+    SLChannelFlow.diagnostics() whenever the host reads them; a row with
+    non-finite entries (its [4] > 0) trips the NaN/Inf guard at once
+    (solver.py:1436-1479)."""
+
+    def __init__(self, u_tau_nominal, factor=2.0, strikes=3):
+        self.u_tau_nominal = u_tau_nominal
+        self.factor = float(factor)
+        self.strikes_needed = int(strikes)
+        self.strikes = 0
+
+    def check(self, row):
+        """row: sequence [max_div, u_tau, cfl, dti, nonfinite].  Returns None,
+        'nonfinite' or 'blowup'."""
+        if float(row[4]) > 0:
+            return "nonfinite"
+        if self.u_tau_nominal is not None and float(row[1]) > self.factor * self.u_tau_nominal:
+            self.strikes += 1
+        else:
+            self.strikes = 0
+        return "blowup" if self.strikes >= self.strikes_needed else None

@@ -171,6 +171,15 @@ class Ensemble:
                                            self._stream()), "slc_ens_stats")
         return out
 
+    def diagnostics(self, mid, out=None):
+        """Device tensor [max|div|, u_tau, CFL, dti, non-finite entries] of the
+        member's current fields (no host synchronisation)."""
+        if out is None:
+            out = torch.empty(5, dtype=torch.float64, device=self.device)
+        _lib.check(self._lib.slc_ens_diagnostics(self._h, int(mid), ctypes.c_void_p(out.data_ptr()),
+                                                 self._stream()), "slc_ens_diagnostics")
+        return out
+
     # -- stepping ------------------------------------------------------------
 
     def step(self):

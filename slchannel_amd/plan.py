@@ -266,6 +266,17 @@ class HipPlan:
         _lib.check(self._lib.slc_bulk_velocity(self._h, _ptr(u), _ptr(out), self._stream()),
                    "slc_bulk_velocity")
 
+    def diagnostics(self, u, v, w, p, dt, out):
+        """out (5 device doubles): max|div|, u_tau, CFL = dti*dt, dti, number
+        of non-finite entries of u, v, w, p (p may be None)."""
+        self._uvw(u, v, w)
+        if p is not None:
+            self._chk(p, self.shapes["p"], "p")
+        self._chk(out, (5,), "out")
+        _lib.check(self._lib.slc_diagnostics(self._h, _ptr(u), _ptr(v), _ptr(w),
+                                             _ptr(p), float(dt),
+                                             _ptr(out), self._stream()), "slc_diagnostics")
+
     def bulk_force(self, u, v, w, U_bulk, dt, out):
         self._uvw(u, v, w)
         self._chk(out, (4,), "out")

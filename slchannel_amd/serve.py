@@ -13,9 +13,18 @@ waiting request can take its place.
 
 Reported per run: arrival, admission and completion times, every step's
 latency (the device time of the batched step it took part in, from
-hipEvents), and optionally its final fields.  Aggregates: p50 / p99 step
-latency, p50 / p99 time in system, member steps per second over the span
-from the first admission to the last completion.
+hipEvents), its final run diagnostics (max|div|, u_tau, CFL, non-finite
+entries; slc_ens_diagnostics), a status, and optionally its final fields.
+Aggregates: p50 / p99 step latency, p50 / p99 time in system, member steps
+per second over the span from the first admission to the last completion.
+
+Guards (the reference run loop's, solver.py:1394-1479, per run): a request
+whose given initial state holds NaN/Inf is rejected without being stepped;
+with ``guard_every = k`` every run's diagnostics are taken every k of its
+steps into pinned host memory and read once their event has completed (no
+host synchronisation), and a run that turned non-finite, or whose u_tau
+exceeded the blow-up factor on three consecutive checks, is stopped and its
+slot released.
 
 The reference has no serving layer (one run per process, solver.py:1280-1500).
 """
@@ -27,7 +36,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from .diagnostics import cfl_dt
+from .diagnostics import BlowupGuard, cfl_dt
 from .ensemble import Ensemble, _INIT_CACHE, _config_grid, initial_fields, member_bytes
 
 
@@ -42,6 +51,8 @@ class Request:
     cfl: float = 3.0
     urms: float = 0.1
     kmax: float = 8.0
+    fields: tuple = None      # (u, v, w[, p]) device tensors: initial state instead of `seed`
+    u_tau_nominal: float = None  # blow-up guard reference (None: non-finite check only)
 
 
 @dataclass
@@ -55,10 +66,19 @@ class Result:
     latencies_ms: list = field(default_factory=list)
     fields: dict = None
     stats: list = None
+    diag: list = None         # [max|div|, u_tau, CFL, dti, non-finite] at completion
+    status: str = "pending"   # ok | rejected_nonfinite | stopped_nonfinite | stopped_blowup
 
 
 def _prepare(req, device):
-    """Initial state and dt of a request on the device (SLChannelFlow.initialize)."""
+    """Initial state and dt of a request on the device (SLChannelFlow.initialize,
+    or the request's own fields)."""
+    if req.fields is not None:
+        u, v, w = (t.to(device=device, dtype=torch.float64).contiguous() for t in req.fields[:3])
+        p = req.fields[3] if len(req.fields) > 3 else None
+        if req.dt is None:
+            raise ValueError(f"request {req.rid}: explicit fields need an explicit dt")
+        return (u, v, w, p), float(req.dt)
     u, v, w, p = initial_fields(req.config, req.seed, urms=req.urms, kmax=req.kmax,
                                 device=device)
     dt = req.dt
@@ -74,7 +94,7 @@ class Server:
     """Continuous-batching loop over one Ensemble."""
 
     def __init__(self, hbm_budget=None, max_members=512, order=6, storage=64, graph=False,
-                 keep_fields=False, device=None):
+                 keep_fields=False, device=None, guard_every=0, blowup_factor=2.0):
         self.ens = Ensemble(order=order, storage=storage, device=device, graph=graph)
         self.device = self.ens.device
         if hbm_budget is None:
@@ -84,6 +104,8 @@ class Server:
         self.max_members = int(max_members)
         self.keep_fields = keep_fields
         self.storage = storage
+        self.guard_every = int(guard_every)
+        self.blowup_factor = float(blowup_factor)
         # admissions (initial fields, dt) run on their own stream so they
         # overlap the batched steps instead of queueing behind them
         self.side = torch.cuda.Stream(self.device)
@@ -121,6 +143,8 @@ class Server:
         active = {}  # member id -> (request, result)
         queued = []  # (start event, end event, results stepped)
         retiring = []  # (event after the copies out, request, result)
+        guards = {}  # member id -> BlowupGuard
+        checks = []  # (event, member id, rid, pinned diagnostics)
         t0 = time.perf_counter()
         n_member_steps = 0
         first_admit = None
@@ -146,10 +170,18 @@ class Server:
                 ta = time.perf_counter()
                 with torch.cuda.stream(self.side):
                     fields, dt = _prepare(req, self.device)
+                    if req.fields is not None and not all(
+                            bool(torch.isfinite(t).all()) for t in fields if t is not None):
+                        res = Result(rid=req.rid, arrival=req.arrival, dt=dt,
+                                     status="rejected_nonfinite")
+                        res.admitted = res.finished = time.perf_counter() - t0
+                        results[req.rid] = res
+                        continue
                     mid = self.ens.add(req.config, dt, fields=fields)
                 host["admit_s"] += time.perf_counter() - ta
                 res = Result(rid=req.rid, admitted=time.perf_counter() - t0, dt=dt)
                 res.arrival = req.arrival
+                guards[mid] = BlowupGuard(req.u_tau_nominal, self.blowup_factor)
                 active[mid] = (req, res)
                 first_admit = res.admitted if first_admit is None else first_admit
                 if log:
@@ -178,15 +210,32 @@ class Server:
             for mid, (req, res) in active.items():
                 res.steps += 1
                 if res.steps >= req.n_steps:
-                    done.append(mid)
+                    done.append((mid, "ok"))
+                elif self.guard_every and res.steps % self.guard_every == 0:
+                    h = torch.empty(5, dtype=torch.float64, pin_memory=True)
+                    h.copy_(self.ens.diagnostics(mid), non_blocking=True)
+                    ev = torch.cuda.Event()
+                    ev.record()
+                    checks.append((ev, mid, req.rid, h))
+            # guard results whose copies have landed (no waiting)
+            while checks and checks[0][0].query():
+                _, mid, rid, h = checks.pop(0)
+                if mid in active and active[mid][0].rid == rid and \
+                        all(m != mid for m, _ in done):
+                    verdict = guards[mid].check(h.tolist())
+                    if verdict is not None:
+                        done.append((mid, "stopped_" + verdict))
             tr = time.perf_counter()
-            for mid in done:
+            for mid, status in done:
                 req, res = active.pop(mid)
+                guards.pop(mid, None)
+                res.status = status
                 # copies are stream-ordered behind the member's last step;
                 # the arena is reused only by work queued after them
                 if self.keep_fields:
                     res.fields = self.ens.get_fields(mid)
                 res.stats = self.ens.stats(mid)
+                res.diag = self.ens.diagnostics(mid)
                 ev = torch.cuda.Event()
                 ev.record()
                 retiring.append((ev, req, res))
@@ -195,12 +244,14 @@ class Server:
                 ev, req, res = retiring.pop(0)
                 res.finished = time.perf_counter() - t0
                 res.stats = res.stats.cpu().tolist()
+                res.diag = res.diag.cpu().tolist()
                 results[req.rid] = res
             host["retire_s"] += time.perf_counter() - tr
         for ev, req, res in retiring:
             ev.synchronize()
             res.finished = time.perf_counter() - t0
             res.stats = res.stats.cpu().tolist()
+            res.diag = res.diag.cpu().tolist()
             results[req.rid] = res
         drain(0)
         t_end = time.perf_counter() - t0

@@ -5,8 +5,11 @@ Keeps the reference's step API -- ``step_sl_bdf2(dt) -> (u_bulk, forcing)``
 on any dt change (solver.py:729-811, 834-844), ``apply_bc_uvw`` and the exact
 bulk-flux constraint (solver.py:694-723) -- and the YAML keys of the hot path
 (grid, domain, flow, boundary_conditions, time.dt, sl.*; docs/CONFIG.md).
-The run loop, statistics, checkpoints and plotting of the reference are not
-part of this layer.
+Restarts read the reference's ``fields.npz`` layout
+(``initialization.field_file``, solver.py:300-347; slchannel_amd.checkpoint)
+and re-bootstrap with one BDF1 step; statistics are
+slchannel_amd.TurbulenceStats.  The run loop and plotting of the reference
+are not part of this layer.
 
 Per step the device does: ghost fill; the fused two-foot BDF2 predictor (far
 pass + near pass per component); ghost fill; three Thomas z-solves; ghost fill;
@@ -16,10 +19,12 @@ synchronisation: u_bulk and forcing come back as 0-D device tensors, as in the
 reference.
 """
 
+import numpy as np
 import torch
 import yaml
 
 from . import env
+from .checkpoint import initialize_flow_from_file, save_flow_fields
 from .grid import generate_grid
 from .initflow import solenoidal_fields
 from .plan import HipPlan, field_shapes
@@ -115,8 +120,18 @@ class SLChannelFlow:
         self.forcing = 0.0
         self.current_step = 0
         self.time = 0.0
+        self.initial_step = 0
+        self.initial_time = 0.0
         self._nclamp = torch.zeros(1, dtype=torch.int64, device=self.device)
         self._bulk = torch.zeros(4, dtype=torch.float64, device=self.device)
+
+        init = cfg.get("initialization") or {}
+        field_file = init.get("field_file")
+        if field_file is not None:
+            if init.get("type", "parabolic") == "interpolate":
+                raise ValueError("initialization.type 'interpolate' (a field from another "
+                                 "grid) is not supported by the HIP layer")
+            self.restart_from(field_file, reset_time=bool(init.get("reset_time", False)))
 
     # ------------------------------------------------------------------
     # fields
@@ -137,6 +152,27 @@ class SLChannelFlow:
             s["p"], dtype=torch.float64, device=self.device)
         self.u_nm1 = self.v_nm1 = self.w_nm1 = None
         self._dt_prev_step = None
+
+    def restart_from(self, field_file, reset_time=False):
+        """Restart from a reference-layout checkpoint (solver.py:338-351):
+        fields, step and time (unless reset_time) and the forcing value; the
+        next step is a BDF1 bootstrap."""
+        u, v, w, p, step, t = initialize_flow_from_file(field_file, device=self.device,
+                                                        reset_time=reset_time)
+        self.set_fields(u, v, w, p)
+        self.initial_step = self.current_step = step
+        self.initial_time = self.time = t
+        try:
+            self.forcing = float(np.load(field_file)["forcing"])
+        except (KeyError, ValueError):
+            self.forcing = 0.0
+
+    def save_checkpoint(self, results_folder, filename="fields.npz", u_tau=0.0):
+        """Write the current state in the reference's fields.npz layout
+        (utils.py:183-215)."""
+        save_flow_fields(self.u, self.v, self.w, self.p, self.z_c, self.z_f, self.Lx, self.Ly,
+                         self.current_step, self.time, u_tau, self.forcing, results_folder,
+                         filename)
 
     def initialize(self, seed, urms=0.1, kmin=1.0, kmax=8.0, spectrum="flat"):
         """Seeded solenoidal Poiseuille + perturbation, boundary-filled and
@@ -239,6 +275,17 @@ class SLChannelFlow:
     def divergence(self):
         out = torch.empty(self.nx, self.ny, self.nz, dtype=torch.float64, device=self.device)
         self.plan.divergence(self.u, self.v, self.w, out)
+        return out
+
+    def diagnostics(self, dt=None, out=None):
+        """Run-loop diagnostics on the device (solver.py:1355-1392, 1436-1479):
+        a (5,) float64 device tensor [max|div|, u_tau, CFL, dti, non-finite
+        entries of u, v, w, p].  No host synchronisation; ``out`` may be a row
+        of a caller-held time-series tensor."""
+        if out is None:
+            out = torch.empty(5, dtype=torch.float64, device=self.device)
+        dt = self.dt if dt is None else dt
+        self.plan.diagnostics(self.u, self.v, self.w, self.p, float(dt or 0.0), out)
         return out
 
     def bulk_velocity(self):

@@ -63,3 +63,37 @@ def test_results_same_alone_and_in_the_mix(check):
               got.stats[:3] == alone.stats[:3], f"{got.stats[:3]} vs {alone.stats[:3]}")
     check("latency percentiles", metrics["p99_step_ms"] >= metrics["p50_step_ms"] > 0,
           f"{metrics}")
+
+
+def test_guards_stop_bad_runs_and_spare_the_rest(check):
+    """The run-loop guards inside the serving loop (solver.py:1394-1479):
+    a request with a NaN in its given initial state is rejected without a
+    step; a run whose u_tau stays above the blow-up factor is stopped after
+    three consecutive checks and its slot released; a healthy run in the
+    same batch finishes and equals its run alone."""
+    from slchannel_amd.ensemble import initial_fields
+    from slchannel_amd.serve import Server, run_alone
+
+    good = _req(0, 24, 16, 24, 2800.0, 6)
+    bad = _req(1, 24, 16, 24, 3000.0, 6)
+    u, v, w, p = initial_fields(bad.config, bad.seed, kmax=4.0)
+    u = u.clone()
+    u[3, 4, 5] = float("nan")
+    bad.fields, bad.dt = (u, v, w, p), 0.02
+    hot = _req(2, 32, 24, 16, 4000.0, 400)
+    hot.u_tau_nominal = 1e-6  # every healthy u_tau is "above 2x nominal"
+    srv = Server(keep_fields=True, guard_every=1)
+    results, _ = srv.run([good, bad, hot])
+    check("NaN initial state rejected unstepped",
+          results[1].status == "rejected_nonfinite" and results[1].steps == 0,
+          f"{results[1].status} steps={results[1].steps}")
+    r2 = results[2]
+    check("blow-up guard stops the run early", r2.status == "stopped_blowup" and
+          3 <= r2.steps < 400, f"{r2.status} steps={r2.steps}")
+    r0 = results[0]
+    alone, _ = run_alone(good)
+    same = all(torch.equal(r0.fields[k], alone.fields[k]) for k in "uvwp")
+    check("healthy run finishes ok and == alone", r0.status == "ok" and same,
+          f"{r0.status} diag={r0.diag}")
+    check("final diagnostics finite, divergence-free",
+          r0.diag[4] == 0.0 and r0.diag[0] < 1e-11 and r0.diag[1] > 0, f"{r0.diag}")
