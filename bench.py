@@ -56,6 +56,10 @@ def parse():
                          "with feet displaced uniformly in +-5 cells; "
                          "4: 48-member uneven ensemble, batched vs one run at a time; "
                          "5: serving loop with Poisson arrivals (p50/p99 step latency)")
+    ap.add_argument("--scheme", default="sl", choices=("sl", "imex"),
+                    help="config 2: time the SL-BDF2 step (default) or the Eulerian IMEX "
+                         "step at the reference's Eulerian CFL 0.28, with both schemes' "
+                         "simulated time per second")
     ap.add_argument("--members", type=int, default=48, help="config 4: ensemble size")
     ap.add_argument("--requests", type=int, default=24, help="config 5: number of requests")
     ap.add_argument("--rate", type=float, default=1.0, help="config 5: arrivals per second")
@@ -107,6 +111,8 @@ def main():
         twin = SLChannelFlow(config=cfg64, device="cuda:0")
         twin.set_fields(flow.u, flow.v, flow.w, flow.p)
 
+    if args.scheme == "imex":
+        return imex_bench(args, flow, dt)
     for _ in range(args.warmup):
         flow.step_sl_bdf2(dt)
     torch.cuda.synchronize()
@@ -193,6 +199,84 @@ def main():
         "kernels_ms": kernels,
         "fp32_storage_rel_err_vs_fp64": fp32_err,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def imex_bench(args, flow, dt_sl, cfl_imex=0.28):
+    """The Eulerian IMEX step (solver.py:1055-1127) on the same state, at the
+    CFL of the reference's Eulerian configuration (configs/
+    re180_eulerian_reference.yaml: CFL_target 0.28), beside the SL-BDF2 step
+    at its dt+ = 0.2, as simulated time per wall second."""
+    import torch
+
+    from slchannel_amd.diagnostics import cfl_dt
+
+    n, nz = flow.nx, flow.nz
+    u0, v0, w0, p0 = (t.clone() for t in (flow.u, flow.v, flow.w, flow.p))
+    dt = cfl_dt(flow, cfl_imex)
+    # the SL step for the comparison (same state)
+    for _ in range(args.warmup):
+        flow.step_sl_bdf2(dt_sl)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ks = max(5, args.steps // 2)
+    for _ in range(ks):
+        flow.step_sl_bdf2(dt_sl)
+    torch.cuda.synchronize()
+    sl_ms = (time.perf_counter() - t0) * 1e3 / ks
+    flow.set_fields(u0, v0, w0, p0)
+    for _ in range(args.warmup):
+        flow.step_imex(dt)
+    torch.cuda.synchronize()
+    plan = flow.plan
+    plan.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ub, _ = flow.step_imex(dt)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = plan.profile_report()
+    plan.profile(False)
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    cnt, tot = prof["imex_rhs"]
+    avg_ms = tot / cnt
+    # algorithmic bytes: u, v, w read once, three rhs arrays written once
+    nelem = ((n + 1) * (n + 2) * (nz + 2) + (n + 2) * (n + 1) * (nz + 2) +
+             (n + 2) * (n + 2) * (nz + 1))
+    bytes_rhs = 2.0 * 8.0 * nelem
+    achieved = bytes_rhs / (avg_ms * 1e-3) / 1e9
+    div = flow.divergence().abs().max().item()
+    kernels = {k: {"launches": c, "avg_ms": round(t / c, 4)} for k, (c, t) in sorted(prof.items())}
+    line = {
+        "metric": "steps/s (Eulerian IMEX step, 256^3 fp64)",
+        "value": round(args.steps / (t1 - t0), 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded solenoidal k^-5/3 perturbation on Poiseuille, projected)",
+        "config": {"workload": f"Eulerian IMEX channel {n}x{n}x{nz} CFL {cfl_imex}",
+                   "nx": n, "ny": n, "nz": nz, "dt": dt},
+        "roofline": {"bound": "hbm", "kernel": "imex_rhs_kernel (advection + xy-diffusion)",
+                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
+                     "bytes_per_launch": bytes_rhs, "avg_ms": round(avg_ms, 4)},
+        "scheme_comparison": {
+            "imex_dt": dt, "imex_cfl": cfl_imex, "imex_ms_per_step": round(ms_step, 4),
+            "imex_sim_time_per_s": round(dt / (ms_step * 1e-3), 5),
+            "sl_dt": dt_sl, "sl_ms_per_step": round(sl_ms, 4),
+            "sl_sim_time_per_s": round(dt_sl / (sl_ms * 1e-3), 5),
+            "sl_over_imex": round((dt_sl / sl_ms) / (dt / ms_step), 3)},
+        "max_div": div,
+        "u_bulk": ub.item(),
+        "kernels_ms": kernels,
+        "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
 

@@ -170,6 +170,22 @@ int slc_project_step(slc_plan* plan, double* u, double* v, double* w, double rhs
  * (slchannel/utils.py:147-163). */
 int slc_bulk_velocity(slc_plan* plan, const double* u, double* out, void* stream);
 
+/* Eulerian IMEX explicit terms, compute_momentum_rhs_fused_imex
+ * (operators.py:639-835; eulerian_triton.py:31-193): divergence-form
+ * advection and the xy Laplacian, rhs = nu (d2x + d2y) - div(f V) on each
+ * component's interior, 0 on every other entry of the ghost-shaped outputs
+ * ru, rv, rw (same shapes as u, v, w).  Each element in the reference's
+ * operation order. */
+int slc_imex_rhs(slc_plan* plan, const double* u, const double* v, const double* w, double* ru,
+                 double* rv, double* rw, void* stream);
+
+/* AB2 explicit update of step_imex (solver.py:1074-1081) over the whole
+ * arrays: f += dt (1.5 r - 0.5 r_prev), or f += dt r when the three r_prev
+ * are null (the first step). */
+int slc_ab2_update(slc_plan* plan, double* u, double* v, double* w, const double* ru,
+                   const double* rv, const double* rw, const double* ru_prev,
+                   const double* rv_prev, const double* rw_prev, double dt, void* stream);
+
 /* Run-loop diagnostics of the current fields, written to out (5 device
  * doubles, no host synchronisation): [0] max|div| (compute_divergence,
  * utils.py:166-180), [1] u_tau (compute_u_tau, utils.py:116-146), [2] the

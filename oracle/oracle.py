@@ -543,6 +543,88 @@ def compute_bulk_velocity(u, cell_vol_ratio, total_volume):
 
 
 # ---------------------------------------------------------------------------
+# Eulerian IMEX right-hand side
+# ---------------------------------------------------------------------------
+
+
+def compute_momentum_rhs_fused_imex(u, v, w, nx, ny, nz, dx, dy, dz_c, dz_f, nu):
+    """operators.py:639-835: divergence-form advection on the MAC grid and the
+    xy Laplacian (no z-diffusion), rhs = diffusion_xy - advection on each
+    component's interior, zero elsewhere; u and v extended periodically by
+    one x-plane / y-plane so every interior face carries its flux."""
+    rhs_u, rhs_v, rhs_w = np.zeros_like(u), np.zeros_like(v), np.zeros_like(w)
+    dx_inv, dy_inv = 1.0 / dx, 1.0 / dy
+    nu_dx2, nu_dy2 = nu * (1.0 / (dx * dx)), nu * (1.0 / (dy * dy))
+    dz_f_inv = (1.0 / np.asarray(dz_f)).reshape(1, 1, -1)
+    dz_c = np.asarray(dz_c)
+    I, J, K = slice(1, nx + 1), slice(1, ny + 1), slice(1, nz + 1)
+    # u (operators.py:679-728)
+    ue = np.concatenate([u, u[1:2]], axis=0)
+    a = 0.5 * (ue[I, J, K] + ue[2:nx + 2, J, K])
+    b = 0.5 * (ue[0:nx, J, K] + ue[I, J, K])
+    duudx = (a * a - b * b) * dx_inv
+    vi = 0.5 * (v[I, J, K] + v[2:nx + 2, J, K])
+    ui = 0.5 * (ue[I, J, K] + ue[I, 2:ny + 2, K])
+    top = vi * ui
+    vi = 0.5 * (v[I, 0:ny, K] + v[2:nx + 2, 0:ny, K])
+    ui = 0.5 * (ue[I, 0:ny, K] + ue[I, J, K])
+    dvudy = (top - vi * ui) * dy_inv
+    wi = 0.5 * (w[I, J, K] + w[2:nx + 2, J, K])
+    ui = 0.5 * (ue[I, J, K] + ue[I, J, 2:nz + 2])
+    top = wi * ui
+    wi = 0.5 * (w[I, J, 0:nz] + w[2:nx + 2, J, 0:nz])
+    ui = 0.5 * (ue[I, J, 0:nz] + ue[I, J, K])
+    dwudz = (top - wi * ui) * dz_f_inv[:, :, 0:nz]
+    adv = duudx + dvudy + dwudz
+    d2x = (ue[2:nx + 2, J, K] - 2.0 * ue[I, J, K] + ue[0:nx, J, K]) * nu_dx2
+    d2y = (u[I, 2:ny + 2, K] - 2.0 * u[I, J, K] + u[I, 0:ny, K]) * nu_dy2
+    rhs_u[I, J, K] = (d2x + d2y) - adv
+    # v (operators.py:734-781)
+    ve = np.concatenate([v, v[:, 1:2]], axis=1)
+    ui = 0.5 * (u[I, J, K] + u[I, 2:ny + 2, K])
+    vi = 0.5 * (ve[I, J, K] + ve[2:nx + 2, J, K])
+    right = ui * vi
+    ui = 0.5 * (u[0:nx, J, K] + u[0:nx, 2:ny + 2, K])
+    vi = 0.5 * (ve[0:nx, J, K] + ve[I, J, K])
+    duvdx = (right - ui * vi) * dx_inv
+    a = 0.5 * (ve[I, J, K] + ve[I, 2:ny + 2, K])
+    b = 0.5 * (ve[I, 0:ny, K] + ve[I, J, K])
+    dvvdy = (a * a - b * b) * dy_inv
+    wi = 0.5 * (w[I, J, K] + w[I, 2:ny + 2, K])
+    vi = 0.5 * (ve[I, J, K] + ve[I, J, 2:nz + 2])
+    top = wi * vi
+    wi = 0.5 * (w[I, J, 0:nz] + w[I, 2:ny + 2, 0:nz])
+    vi = 0.5 * (ve[I, J, 0:nz] + ve[I, J, K])
+    dwvdz = (top - wi * vi) * dz_f_inv[:, :, 0:nz]
+    adv = duvdx + dvvdy + dwvdz
+    d2x = (v[2:nx + 2, J, K] - 2.0 * v[I, J, K] + v[0:nx, J, K]) * nu_dx2
+    d2y = (ve[I, 2:ny + 2, K] - 2.0 * ve[I, J, K] + ve[I, 0:ny, K]) * nu_dy2
+    rhs_v[I, J, K] = (d2x + d2y) - adv
+    # w (operators.py:787-833), interior faces 1..nz-1
+    Kw = slice(1, nz)
+    ui = 0.5 * (u[I, J, Kw] + u[I, J, 2:nz + 1])
+    wi = 0.5 * (w[I, J, Kw] + w[2:nx + 2, J, Kw])
+    right = ui * wi
+    ui = 0.5 * (u[0:nx, J, Kw] + u[0:nx, J, 2:nz + 1])
+    wi = 0.5 * (w[0:nx, J, Kw] + w[I, J, Kw])
+    duwdx = (right - ui * wi) * dx_inv
+    vi = 0.5 * (v[I, J, Kw] + v[I, J, 2:nz + 1])
+    wi = 0.5 * (w[I, J, Kw] + w[I, 2:ny + 2, Kw])
+    top = vi * wi
+    vi = 0.5 * (v[I, 0:ny, Kw] + v[I, 0:ny, 2:nz + 1])
+    wi = 0.5 * (w[I, 0:ny, Kw] + w[I, J, Kw])
+    dvwdy = (top - vi * wi) * dy_inv
+    a = 0.5 * (w[I, J, Kw] + w[I, J, 2:nz + 1])
+    b = 0.5 * (w[I, J, 0:nz - 1] + w[I, J, Kw])
+    dwwdz = (a * a - b * b) / dz_c[1:nz].reshape(1, 1, -1)
+    adv = duwdx + dvwdy + dwwdz
+    d2x = (w[2:nx + 2, J, Kw] - 2.0 * w[I, J, Kw] + w[0:nx, J, Kw]) * nu_dx2
+    d2y = (w[I, 2:ny + 2, Kw] - 2.0 * w[I, J, Kw] + w[I, 0:ny, Kw]) * nu_dy2
+    rhs_w[I, J, Kw] = (d2x + d2y) - adv
+    return rhs_u, rhs_v, rhs_w
+
+
+# ---------------------------------------------------------------------------
 # Run-loop diagnostics
 # ---------------------------------------------------------------------------
 
@@ -679,17 +761,39 @@ class ChannelOracle:
         u_bulk_current = compute_bulk_velocity(self.u, self.cell_vol_ratio, self.total_volume)
         return u_bulk_current, self.forcing
 
-    def _zsolve_all(self, dt_z):
+    def _zsolve_all(self, dt_z, theta=1.0):
         nx, ny, nz = self.nx, self.ny, self.nz
         self.u = solve_implicit_diffusion_u(self.u, dt_z, nx, ny, nz, self.dz_c, self.dz_f,
-                                            self.nu, theta=1.0,
+                                            self.nu, theta=theta,
                                             top_wall_bc_type=self.top_wall_bc_type)
         self.v = solve_implicit_diffusion_v(self.v, dt_z, nx, ny, nz, self.dz_c, self.dz_f,
-                                            self.nu, theta=1.0,
+                                            self.nu, theta=theta,
                                             top_wall_bc_type=self.top_wall_bc_type)
         self.w = solve_implicit_diffusion_w(self.w, dt_z, nx, ny, nz, self.dz_c, self.dz_f,
-                                            self.nu, theta=1.0)
+                                            self.nu, theta=theta)
         self.apply_bc_uvw()
+
+    def step_imex(self, dt):
+        """solver.py:1055-1127: AB2 explicit advection + xy-diffusion (forward
+        Euler on the first step), Crank-Nicolson z-diffusion, projection with
+        dt, bulk shift."""
+        self.apply_bc_uvw()
+        rhs = compute_momentum_rhs_fused_imex(self.u, self.v, self.w, self.nx, self.ny, self.nz,
+                                              self.dx, self.dy, self.dz_c, self.dz_f, self.nu)
+        prev = getattr(self, "_imex_prev", None)
+        if prev is None:
+            self.u = self.u + dt * rhs[0]
+            self.v = self.v + dt * rhs[1]
+            self.w = self.w + dt * rhs[2]
+        else:
+            self.u = self.u + dt * (1.5 * rhs[0] - 0.5 * prev[0])
+            self.v = self.v + dt * (1.5 * rhs[1] - 0.5 * prev[1])
+            self.w = self.w + dt * (1.5 * rhs[2] - 0.5 * prev[2])
+        self._imex_prev = rhs
+        self.apply_bc_uvw()
+        self._zsolve_all(dt, theta=0.5)
+        self._project(dt, dt)
+        return self._apply_bulk_forcing(dt)
 
     def _project(self, rhs_scale, dt_proj):
         nx, ny, nz = self.nx, self.ny, self.nz

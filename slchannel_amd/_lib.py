@@ -19,7 +19,8 @@ EXPORTS = (
     "slc_plan_workspace_bytes", "slc_apply_bc", "slc_departure", "slc_gather",
     "slc_stencil", "slc_locate_z", "slc_advect", "slc_bdf1_predict", "slc_bdf2_predict",
     "slc_zsolve", "slc_divergence", "slc_poisson", "slc_project", "slc_project_step",
-    "slc_bulk_velocity", "slc_bulk_force", "slc_diagnostics", "slc_set_option",
+    "slc_bulk_velocity", "slc_bulk_force", "slc_diagnostics", "slc_imex_rhs",
+    "slc_ab2_update", "slc_set_option",
     "slc_profile_enable", "slc_profile_report",
     "slc_ens_create", "slc_ens_destroy", "slc_ens_member_bytes", "slc_ens_add",
     "slc_ens_remove", "slc_ens_count", "slc_ens_bytes", "slc_ens_set_fields",
@@ -82,6 +83,8 @@ def load():
         "slc_bulk_velocity": ([vp, vp, vp, vp], i),
         "slc_bulk_force": ([vp, vp, vp, vp, d, d, vp, vp], i),
         "slc_diagnostics": ([vp, vp, vp, vp, vp, d, vp, vp], i),
+        "slc_imex_rhs": ([vp, vp, vp, vp, vp, vp, vp, vp], i),
+        "slc_ab2_update": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, d, vp], i),
         "slc_set_option": ([vp, i, i], i),
         "slc_profile_enable": ([vp, i], i),
         "slc_profile_report": ([vp, ctypes.c_char_p, ctypes.c_int64], i),

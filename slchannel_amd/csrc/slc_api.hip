@@ -103,35 +103,6 @@ int check_plan(slc_plan* p) {
 
 inline hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
 
-hipEvent_t pooled_event(slc_plan* p) {
-    if (!p->pool.empty()) {
-        hipEvent_t e = p->pool.back();
-        p->pool.pop_back();
-        return e;
-    }
-    hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
-    return e;
-}
-
-// Records events around one launch when profiling is on.
-struct Timed {
-    slc_plan* p;
-    const char* label;
-    hipStream_t s;
-    hipEvent_t a = nullptr;
-    Timed(slc_plan* p_, const char* l, void* st) : p(p_), label(l), s(S(st)) {
-        if (p->prof && (a = pooled_event(p))) (void)hipEventRecord(a, s);
-    }
-    ~Timed() {
-        if (!a) return;
-        hipEvent_t b = pooled_event(p);
-        if (!b) return;
-        (void)hipEventRecord(b, s);
-        p->recs.push_back({label, a, b});
-    }
-};
-
 const char* kLabel[3][6] = {
     {"bdf2_far_u", "bdf2_near_u", "bdf1_u", "advect_u", "gather_u", "departure_u"},
     {"bdf2_far_v", "bdf2_near_v", "bdf1_v", "advect_v", "gather_v", "departure_v"},
@@ -246,6 +217,12 @@ int slc::plan_create(const slc_grid_desc* d, int device, bool spectral, slc_plan
     UP(denom_table(zf, P), g.dinv_fP);
     UP(dzc, p->d_dzc);
     UP(dzf, p->d_dzf);
+    {
+        // 1 / dz_f as the reference's tensor reciprocal (operators.py:669)
+        std::vector<double> dzfi(dzf.size());
+        for (size_t k = 0; k < dzf.size(); ++k) dzfi[k] = 1.0 / dzf[k];
+        UP(dzfi, p->d_dzf_inv);
+    }
     UP(cv, p->d_cv);
 
     // Poisson tables (projection.py:65-103) as 1-D factors

@@ -75,7 +75,8 @@ struct EnsMember {
     int* state;
     double dt, U_bulk, total_volume, inv_n;
     int npart;
-    int zcols[3];
+    int zcols[3];  // columns per z-solve work unit
+    int zch[3];    // register z-solve unknowns per lane (0: LDS column path)
     long nghost[4];  // ghost entries of u, v, w, p
 };
 

@@ -345,7 +345,10 @@ int rebuild(slc_ens* e) {
         M.total_volume = pl->total_volume;
         M.inv_n = 1.0 / ((double)m->nx * (double)m->ny);
         M.npart = pl->max_partials;
-        for (int k = 0; k < 3; ++k) M.zcols[k] = zsolve_cols(k, m->nz);
+        for (int k = 0; k < 3; ++k) {
+            M.zch[k] = zsolve_scan_ch(k, m->nz);
+            M.zcols[k] = M.zch[k] > 0 ? kZScanCols : zsolve_cols(k, m->nz);
+        }
         for (int k = 0; k < 4; ++k) M.nghost[k] = ghost_count(k, m->nx, m->ny, m->nz);
         // work units
         unsigned units[EK_COUNT];
@@ -359,8 +362,9 @@ int rebuild(slc_ens* e) {
             units[EK_TILE0 + k] = tiles_of(m, k, e->tile_shape);
             const long ncols = (long)m->nx * m->ny;
             units[EK_ZS0 + k] = (unsigned)((ncols + M.zcols[k] - 1) / M.zcols[k]);
-            e->zs_lds[k] = std::max(e->zs_lds[k], (size_t)M.zcols[k] * zsolve_ld(k, m->nz) *
-                                                      sizeof(double));
+            if (M.zch[k] == 0)
+                e->zs_lds[k] = std::max(e->zs_lds[k], (size_t)M.zcols[k] *
+                                                          zsolve_ld(k, m->nz) * sizeof(double));
         }
         units[EK_PLANE] = (unsigned)plane_tiles(m->nx, m->ny, m->nz);
         units[EK_THOMAS] = ublocks((long)m->nx * nky);
@@ -510,7 +514,7 @@ int64_t member_bytes(int nx, int ny, int nz, int order, int storage) {
     b += spectral_bytes(nx, ny, nz);
     // tables: z (2 P + 4 columns of nz+2), arrival, Poisson, z-solve factors,
     // bulk and diagnostics partials
-    b += 8L * (nz + 2) * (2 * order + 8 + 18) + 8L * (nx + ny) * 4 + 4L * 3 * nz +
+    b += 8L * (nz + 2) * (2 * order + 8 + 19) + 8L * (nx + ny) * 4 + 4L * 3 * nz +
          8L * bulk_blocks(nx, ny, nz) + 8L * 5 * kDiagMaxBlocks + 4096;
     return b;
 }

@@ -135,8 +135,14 @@ __global__ __launch_bounds__(256) void ens_zsolve_kernel(ENS_PARAMS) {
         const int st = *M.state;
         const int nz = M.g.nz;
         const double* tab = M.ztab + (ens_bdf2(st) ? 9 : 0) * (long)(nz + 2) + C * 3L * (nz + 2);
-        zsolve_body<C>(M.g, M.fld[ens_c(st)][C], tab, 0.0, M.dzc, M.dzf, M.zcols[C], u, lds,
-                       cbase);
+        if (M.zch[C] > 0) {
+            const long col = (long)u * kZScanCols + (threadIdx.x >> 6);
+            if (col < (long)M.g.nx * M.g.ny)
+                zsolve_scan_any<C>(M.g, M.fld[ens_c(st)][C], tab, col, M.zch[C]);
+        } else {
+            zsolve_body<C>(M.g, M.fld[ens_c(st)][C], tab, 0.0, M.dzc, M.dzf, M.zcols[C], u, lds,
+                           cbase);
+        }
     });
 }
 

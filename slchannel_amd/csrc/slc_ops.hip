@@ -147,6 +147,34 @@ __global__ __launch_bounds__(256) void zsolve_kernel(Grid g, double* __restrict_
     zsolve_body<C>(g, F, tab, beta, dz_c, dz_f, kZCols, blockIdx.x, lds, cbase);
 }
 
+template <int C, int CH>
+__global__ __launch_bounds__(256) void zsolve_scan_kernel(Grid g, double* __restrict__ F,
+                                                          const double* __restrict__ tab,
+                                                          double beta,
+                                                          const double* __restrict__ dz_c,
+                                                          const double* __restrict__ dz_f) {
+    const long col = (long)blockIdx.x * kZScanCols + (threadIdx.x >> 6);
+    if (col < (long)g.nx * g.ny) zsolve_scan_col<C, CH>(g, F, tab, col, beta, dz_c, dz_f);
+}
+
+template <int C>
+void launch_zsolve_scan(const Grid& g, double* f, const double* tab, int ch, double beta,
+                        const double* dz_c, const double* dz_f, hipStream_t s) {
+    const long ncols = (long)g.nx * g.ny;
+    const dim3 grid((unsigned)((ncols + kZScanCols - 1) / kZScanCols)), blk(256);
+#define SLC_ZSCAN(CHV)                                                                      \
+    hipLaunchKernelGGL((zsolve_scan_kernel<C, CHV>), grid, blk, 0, s, g, f, tab, beta, dz_c, \
+                       dz_f)
+    switch (ch) {
+        case 1: SLC_ZSCAN(1); break;
+        case 2: SLC_ZSCAN(2); break;
+        case 4: SLC_ZSCAN(4); break;
+        case 8: SLC_ZSCAN(8); break;
+        default: SLC_ZSCAN(16); break;
+    }
+#undef SLC_ZSCAN
+}
+
 hipError_t run_zsolve(const Grid& g, int comp, double* f, double dt, double theta,
                       const double* dz_c, const double* dz_f, double* tab, bool refactor,
                       hipStream_t s) {
@@ -154,6 +182,13 @@ hipError_t run_zsolve(const Grid& g, int comp, double* f, double dt, double thet
         hipLaunchKernelGGL(zfactor_kernel, dim3(1), dim3(64), 0, s, g, comp, dt, theta, dz_c,
                            dz_f, tab);
     double beta = theta < 1.0 ? (1.0 - theta) * dt * g.nu : 0.0;
+    const int ch = zsolve_scan_ch(comp, g.nz);
+    if (ch > 0) {
+        if (comp == 0) launch_zsolve_scan<0>(g, f, tab, ch, beta, dz_c, dz_f, s);
+        else if (comp == 1) launch_zsolve_scan<1>(g, f, tab, ch, beta, dz_c, dz_f, s);
+        else launch_zsolve_scan<2>(g, f, tab, ch, beta, dz_c, dz_f, s);
+        return hipGetLastError();
+    }
     long ncols = (long)g.nx * g.ny;
     const int cols = zsolve_cols(comp, g.nz);
     if (cols < 1) return hipErrorInvalidValue;  // nz beyond ~6000: column does not fit LDS

@@ -230,6 +230,173 @@ __device__ inline void zsolve_body(const Grid& g, double* __restrict__ F,
     }
 }
 
+// z-solve of the theta-method, one column per wavefront.  Lane
+// l holds unknowns [l CH, (l+1) CH) in registers.  Forward and back
+// substitution of the shared factorisation are first-order linear
+// recurrences: each lane runs its stretch as an affine map of the value
+// entering it (the same operations as the sequential sweep where that value
+// is known: lane 0 forward, the lane holding the last unknown backward), the
+// 64 maps are composed with a Kogge-Stone scan (6 shuffle steps), and each
+// lane applies its entering value.  Every unknown is read and written once;
+// a wavefront's loads and stores cover its column's n contiguous doubles.
+// Differs from the sequential sweep by rounding only (tridiagonal solves
+// carry the 1e-13 tolerance, not bit-exactness).
+constexpr int kZScanMaxCh = 16;
+constexpr int kZScanCols = 4;  // columns (wavefronts) per 256-thread workgroup
+
+// unknowns per lane (power of two), 0 when the column is too long for the
+// register path (n > 64 kZScanMaxCh)
+__host__ __device__ inline int zsolve_scan_ch(int comp, int nz) {
+    const int n = comp == 2 ? nz - 1 : nz;
+    int ch = 1;
+    while (ch * 64 < n) ch *= 2;
+    return ch <= kZScanMaxCh ? ch : 0;
+}
+
+// Explicit part of the theta-method right-hand side, x_j + beta Dzz x_j, on
+// the lane's unknowns (the formulas and operation order of zsolve_body, from
+// operators.py:446-466 for u, v and 609-623 for w).  p[i] holds unknown
+// j0 + i and, for j0 + i == n, the entry just above the last unknown (the
+// top ghost of u, v; the top wall face of w); s0 is the bottom ghost / wall
+// face.  Neighbours across lanes come by shuffles (all lanes take part).
+template <int C, int CH>
+__device__ inline void zsolve_explicit(const Grid& g, double* p, int j0, int n, double beta,
+                                       const double* __restrict__ dz_c,
+                                       const double* __restrict__ dz_f, double s0) {
+    double ext[CH + 3];  // unknowns j0-2 .. j0+CH
+    ext[1] = __shfl_up(p[CH - 1], 1);
+    ext[0] = CH >= 2 ? __shfl_up(p[CH >= 2 ? CH - 2 : 0], 1) : __shfl_up(p[0], 2);
+    ext[CH + 2] = __shfl_down(p[0], 1);
+#pragma unroll
+    for (int i = 0; i < CH; ++i) ext[i + 2] = p[i];
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+        const int j = j0 + i;
+        if (j >= n) continue;
+        double d2;
+        if (C < 2) {
+            const double lo = ext[i + 1], mid = ext[i + 2], hi = ext[i + 3];
+            const double dzl = dz_c[j], dzr = dz_c[j + 1], dzc = dz_f[j];
+            if (j == 0) {
+                d2 = ((hi - mid) / dzr - 2.0 * mid / dzl) / dzc;
+            } else if (j == n - 1) {
+                if (g.top_neumann) d2 = (-(mid - lo) / dzl) / dzc;
+                else d2 = (-2.0 * mid / dzr - (mid - lo) / dzl) / dzc;
+            } else {
+                d2 = ((hi - mid) / dzr - (mid - lo) / dzl) / dzc;
+            }
+            p[i] = mid + beta * d2;
+        } else {
+            // centred on face j (value b), stored to face j + 1 (value c)
+            const double c = ext[i + 2];
+            const double b = (j == 0) ? s0 : ext[i + 1];
+            const double a = (j == 1) ? s0 : ext[i];
+            const double dzl = dz_f[j], dzr = dz_f[j + 1], dzc = dz_c[j + 1];
+            if (j == 0) d2 = ((c - b) / dzr - b / dzl) / dzc;
+            else if (j == n - 1) d2 = (-b / dzr - (b - a) / dzl) / dzc;
+            else d2 = ((c - b) / dzr - (b - a) / dzl) / dzc;
+            p[i] = c + beta * d2;
+        }
+    }
+}
+
+template <int C, int CH>
+__device__ inline void zsolve_scan_col(const Grid& g, double* __restrict__ F,
+                                       const double* __restrict__ tab, long col,
+                                       double beta = 0.0,
+                                       const double* __restrict__ dz_c = nullptr,
+                                       const double* __restrict__ dz_f = nullptr) {
+    const int lane = threadIdx.x & 63;
+    const int n = (C == 2) ? g.nz - 1 : g.nz;
+    const unsigned ug = (unsigned)col;
+    const int I = 1 + (int)(ug / (unsigned)g.ny), J = 1 + (int)(ug % (unsigned)g.ny);
+    double* Fc = F + ((long)I * Comp<C>::SY(g) + J) * Comp<C>::SZ(g) + 1;
+    const double* A = tab;
+    const double* CP = tab + n;
+    const double* INV = tab + 2 * n;
+    const int j0 = lane * CH;
+    double p[CH], q[CH];
+#pragma unroll
+    for (int i = 0; i < CH; ++i) p[i] = (j0 + i <= n) ? Fc[j0 + i] : 0.0;
+    if (beta != 0.0) zsolve_explicit<C, CH>(g, p, j0, n, beta, dz_c, dz_f, Fc[-1]);
+    // forward: dp_j = p_j + q_j y with y = dp_{j0-1}
+    double pr = 0.0, qr = 1.0;
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+        const int j = j0 + i;
+        if (j < n) {
+            if (j == 0) {
+                pr = p[i] * INV[0];
+                qr = 0.0;
+            } else {
+                const double a = A[j], r = INV[j];
+                pr = (p[i] - a * pr) * r;
+                qr = -(a * qr) * r;
+            }
+        }
+        p[i] = pr;
+        q[i] = qr;
+    }
+    double P = pr, Q = qr;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double Pp = __shfl_up(P, off), Qp = __shfl_up(Q, off);
+        if (lane >= off) {
+            P = P + Q * Pp;
+            Q = Q * Qp;
+        }
+    }
+    double y = __shfl_up(P, 1);
+    if (lane == 0) y = 0.0;
+#pragma unroll
+    for (int i = 0; i < CH; ++i) p[i] = p[i] + q[i] * y;
+    // backward: x_j = r_j + t_j z with z = x_{j0+CH}
+    double rr = 0.0, tr = 1.0;
+#pragma unroll
+    for (int i = CH - 1; i >= 0; --i) {
+        const int j = j0 + i;
+        if (j < n) {
+            if (j == n - 1) {
+                rr = p[i];
+                tr = 0.0;
+            } else {
+                const double c = CP[j];
+                rr = p[i] - c * rr;
+                tr = -(c * tr);
+            }
+        }
+        p[i] = rr;
+        q[i] = tr;
+    }
+    double R = rr, T = tr;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double Rn = __shfl_down(R, off), Tn = __shfl_down(T, off);
+        if (lane + off < 64) {
+            R = R + T * Rn;
+            T = T * Tn;
+        }
+    }
+    double z = __shfl_down(R, 1);
+    if (lane == 63) z = 0.0;
+#pragma unroll
+    for (int i = 0; i < CH; ++i)
+        if (j0 + i < n) Fc[j0 + i] = p[i] + q[i] * z;
+}
+
+// Runtime CH dispatch of zsolve_scan_col (ch from zsolve_scan_ch), theta = 1.
+template <int C>
+__device__ inline void zsolve_scan_any(const Grid& g, double* __restrict__ F,
+                                       const double* __restrict__ tab, long col, int ch) {
+    switch (ch) {
+        case 1: zsolve_scan_col<C, 1>(g, F, tab, col); break;
+        case 2: zsolve_scan_col<C, 2>(g, F, tab, col); break;
+        case 4: zsolve_scan_col<C, 4>(g, F, tab, col); break;
+        case 8: zsolve_scan_col<C, 8>(g, F, tab, col); break;
+        default: zsolve_scan_col<C, 16>(g, F, tab, col); break;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // divergence -> plane-major (k, i, j) Poisson right-hand side
 // ---------------------------------------------------------------------------

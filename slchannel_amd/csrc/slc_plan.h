@@ -58,6 +58,7 @@ struct slc_plan {
     int64_t bytes = 0;
     const double* d_dzc = nullptr;
     const double* d_dzf = nullptr;
+    const double* d_dzf_inv = nullptr;  // 1 / dz_f
     const double* d_cv = nullptr;
     double* planes = nullptr;   // (nz, nx, ny) real
     double* dhat = nullptr;     // (nz, nx, nky) complex
@@ -101,5 +102,39 @@ constexpr int kDiagMaxBlocks = 2048;
 // Plan construction; spectral = false skips the Poisson planes, spectra and
 // rocFFT plans (members of an ensemble share per-size-class ones).
 int plan_create(const slc_grid_desc* d, int device, bool spectral, slc_plan** out);
+
+}  // namespace slc
+
+namespace slc {
+
+inline hipEvent_t pooled_event(slc_plan* p) {
+    if (!p->pool.empty()) {
+        hipEvent_t e = p->pool.back();
+        p->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Records events around one launch when profiling is on (slc_profile_enable).
+struct Timed {
+    slc_plan* p;
+    const char* label;
+    hipStream_t s;
+    hipEvent_t a = nullptr;
+    Timed(slc_plan* p_, const char* l, void* st)
+        : p(p_), label(l), s(static_cast<hipStream_t>(st)) {
+        if (p->prof && (a = pooled_event(p))) (void)hipEventRecord(a, s);
+    }
+    ~Timed() {
+        if (!a) return;
+        hipEvent_t b = pooled_event(p);
+        if (!b) return;
+        (void)hipEventRecord(b, s);
+        p->recs.push_back({label, a, b});
+    }
+};
 
 }  // namespace slc

@@ -266,6 +266,24 @@ class HipPlan:
         _lib.check(self._lib.slc_bulk_velocity(self._h, _ptr(u), _ptr(out), self._stream()),
                    "slc_bulk_velocity")
 
+    def imex_rhs(self, u, v, w, ru, rv, rw):
+        """compute_momentum_rhs_fused_imex into ru, rv, rw (shapes of u, v, w)."""
+        self._uvw(u, v, w)
+        self._uvw(ru, rv, rw, "_rhs")
+        _lib.check(self._lib.slc_imex_rhs(self._h, _ptr(u), _ptr(v), _ptr(w), _ptr(ru), _ptr(rv),
+                                          _ptr(rw), self._stream()), "slc_imex_rhs")
+
+    def ab2_update(self, u, v, w, r, r_prev, dt):
+        """u, v, w += dt (1.5 r - 0.5 r_prev); += dt r when r_prev is None."""
+        self._uvw(u, v, w)
+        self._uvw(*r, "_rhs")
+        if r_prev is not None:
+            self._uvw(*r_prev, "_prev")
+        rp = [_ptr(t) for t in r_prev] if r_prev is not None else [_ptr(None)] * 3
+        _lib.check(self._lib.slc_ab2_update(self._h, _ptr(u), _ptr(v), _ptr(w),
+                                            *[_ptr(t) for t in r], *rp, float(dt),
+                                            self._stream()), "slc_ab2_update")
+
     def diagnostics(self, u, v, w, p, dt, out):
         """out (5 device doubles): max|div|, u_tau, CFL = dti*dt, dti, number
         of non-finite entries of u, v, w, p (p may be None)."""

@@ -117,6 +117,7 @@ class SLChannelFlow:
         self.u_nm1 = self.v_nm1 = self.w_nm1 = None
         self._star = None
         self._dt_prev_step = None
+        self._imex_cur = self._imex_prev = None  # IMEX rhs buffers (step_imex)
         self.forcing = 0.0
         self.current_step = 0
         self.time = 0.0
@@ -152,6 +153,7 @@ class SLChannelFlow:
             s["p"], dtype=torch.float64, device=self.device)
         self.u_nm1 = self.v_nm1 = self.w_nm1 = None
         self._dt_prev_step = None
+        self._imex_prev = None
 
     def restart_from(self, field_file, reset_time=False):
         """Restart from a reference-layout checkpoint (solver.py:338-351):
@@ -210,12 +212,12 @@ class SLChannelFlow:
         self.forcing = self._bulk[1].clone()
         return self._bulk[2].clone(), self.forcing
 
-    def _implicit_and_project(self, dt_z, rhs_scale, dt_proj):
+    def _implicit_and_project(self, dt_z, rhs_scale, dt_proj, theta=1.0):
         plan = self.plan
         plan.apply_bc(self.u, self.v, self.w)
-        plan.zsolve("u", self.u, dt_z, 1.0)
-        plan.zsolve("v", self.v, dt_z, 1.0)
-        plan.zsolve("w", self.w, dt_z, 1.0)
+        plan.zsolve("u", self.u, dt_z, theta)
+        plan.zsolve("v", self.v, dt_z, theta)
+        plan.zsolve("w", self.w, dt_z, theta)
         plan.apply_bc(self.u, self.v, self.w)
         plan.project_step(self.u, self.v, self.w, rhs_scale, dt_proj, self.p)
 
@@ -264,6 +266,40 @@ class SLChannelFlow:
             self.sl.n_clamped_last = self._nclamp[0].clone()
             self._dt_prev_step = dt
             out = self._apply_bulk_forcing(dt)
+        self.current_step += 1
+        self.time += dt
+        return out
+
+    # ------------------------------------------------------------------
+    # Eulerian IMEX reference scheme (solver.py:1024-1127)
+    # ------------------------------------------------------------------
+
+    def compute_momentum_rhs_explicit_imex(self):
+        """solver.py:1024-1053: (rhs_u, rhs_v, rhs_w) = diffusion_xy - advection,
+        new device tensors."""
+        s = self.shapes
+        r = [torch.empty(s[c], dtype=torch.float64, device=self.device) for c in "uvw"]
+        self.plan.imex_rhs(self.u, self.v, self.w, *r)
+        return tuple(r)
+
+    def step_imex(self, dt):
+        """solver.py:1055-1127: AB2 explicit advection + xy-diffusion (forward
+        Euler while there is no previous rhs), Crank-Nicolson z-diffusion,
+        projection with dt, bulk shift.  Returns (u_bulk, forcing) as 0-D
+        device tensors."""
+        dt = float(dt)
+        self.apply_bc_uvw()
+        s = self.shapes
+        if self._imex_cur is None:
+            self._imex_cur = [torch.empty(s[c], dtype=torch.float64, device=self.device)
+                              for c in "uvw"]
+        self.plan.imex_rhs(self.u, self.v, self.w, *self._imex_cur)
+        prev = self._imex_prev
+        self.plan.ab2_update(self.u, self.v, self.w, self._imex_cur, prev, dt)
+        # the current rhs becomes the previous one (buffers swapped, no copy)
+        self._imex_prev, self._imex_cur = self._imex_cur, prev
+        self._implicit_and_project(dt, dt, dt, theta=0.5)
+        out = self._apply_bulk_forcing(dt)
         self.current_step += 1
         self.time += dt
         return out
