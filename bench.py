@@ -50,7 +50,7 @@ def parse():
                     help="bits of the gathered field copies: 64 (default) or 32 "
                          "(FP32-storage variant; adds an FP64 twin run and reports the "
                          "per-field difference)")
-    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5),
+    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5, 6),
                     help="2: 256^3 channel (default, the headline workload); "
                          "3: 1024x1024x512 large box, CFL 4, plus the isolated gather "
                          "with feet displaced uniformly in +-5 cells; "
@@ -91,6 +91,8 @@ def main():
         return ensemble_bench(args)
     if args.config == 5:
         return serve_bench(args)
+    if args.config == 6:
+        return stats_bench(args)
     n, nz = args.n, args.nz or args.n
     Re_b, Re_tau, dtp = 2792.8, 180.0, 0.2
     cfg = {
@@ -277,6 +279,84 @@ def imex_bench(args, flow, dt_sl, cfl_imex=0.28):
         "u_bulk": ub.item(),
         "kernels_ms": kernels,
         "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def stats_bench(args):
+    """On-device turbulence statistics (TurbulenceStats.accumulate_statistics,
+    include/slc_stats.h) at n^3 with the primary z+ = 15 plane pair and three
+    extra spectra planes (z+ = 5, 30, 100): snapshots per second, fields
+    resident in HBM.  Algorithmic bytes per snapshot: the two plane passes
+    read u, v, w once each (2 x 24 B per grid point)."""
+    import numpy as np
+    import torch
+
+    from slchannel_amd.grid import generate_grid
+    from slchannel_amd.turbstats import TurbulenceStats
+
+    n = args.n
+    nz = args.nz or n
+    Lx, Ly, Lz = 4 * math.pi, 2 * math.pi, 2.0
+    z_f, z_c, dz_f, dz_c = generate_grid(1.5, nz, Lz, stretching_type="symmetric")
+    nu = 1.0 / 5600.0
+    ts = TurbulenceStats(n, n, nz, Lx, Ly, Lz, z_c, z_f, dz_c, dz_f, Lx / n, Ly / n, nu, 180.0,
+                         device="cuda", spectra_z_planes=[5.0, 30.0, 100.0])
+    g = torch.Generator(device="cuda").manual_seed(1)
+    f64 = dict(dtype=torch.float64, device="cuda", generator=g)
+    u = 1.0 + 0.1 * torch.randn(n + 1, n + 2, nz + 2, **f64)
+    v = 0.1 * torch.randn(n + 2, n + 1, nz + 2, **f64)
+    w = 0.1 * torch.randn(n + 2, n + 2, nz + 1, **f64)
+    for _ in range(args.warmup):
+        ts.accumulate_statistics(u, v, w)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        ts.accumulate_statistics(u, v, w)
+    e1.record()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = e0.elapsed_time(e1) / args.steps
+    nfield = ((n + 1) * (n + 2) * (nz + 2) + (n + 2) * (n + 1) * (nz + 2) +
+              (n + 2) * (n + 2) * (nz + 1))
+    bytes_snap = 2.0 * 8.0 * nfield
+    achieved = bytes_snap / (ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle.turbstats import StatsOracle
+
+        orc = StatsOracle(n, n, nz, planes=[(ts.k_bot, ts.k_top)] +
+                          [(p["k_bot"], p["k_top"]) for p in ts.extra_planes])
+        uh, vh, wh = (np.ascontiguousarray(t.cpu().numpy()) for t in (u, v, w))
+        c0 = time.perf_counter()
+        orc.accumulate(uh, vh, wh)
+        c1 = time.perf_counter()
+        cpu = {"value": round(1.0 / (c1 - c0), 4), "unit": "snapshots/s", "cores": 1,
+               "kind": "port", "sample": f"one snapshot at {n}x{n}x{nz} (numpy oracle, "
+                                         "oracle/turbstats.py)"}
+    line = {
+        "metric": f"snapshots/s (turbulence statistics accumulate, {n}x{n}x{nz} fp64)",
+        "value": round(args.steps / (t1 - t0), 3),
+        "unit": "snapshots/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random fields)",
+        "config": {"workload": f"TurbulenceStats.accumulate_statistics {n}x{n}x{nz}, "
+                               "4 spectra plane pairs", "nx": n, "ny": n, "nz": nz},
+        "roofline": {"bound": "hbm", "kernel": "slc_stats_accumulate (two plane passes, "
+                                               "reduces, spectra)",
+                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
+                     "bytes_per_launch": bytes_snap, "avg_ms": round(ms, 4)},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 

@@ -20,7 +20,7 @@
 //
 // Layout of the passes: a workgroup is 64 z lanes x 4 column groups.  The z
 // lanes make every load a coalesced 512-byte row segment; each thread walks
-// an x-run of 32 cells for 4 y-rows, carrying the previous u face in a
+// an x-run of 16 cells for 4 y-rows, carrying the previous u face in a
 // register so the cell-centred u costs one load.  Partials go to
 // part[(chunk * NQ + q) * (nz + 1) + k] and are summed in chunk order by the
 // reduce kernel, so the result does not depend on scheduling.
@@ -38,7 +38,7 @@ namespace {
 
 constexpr int kLanes = 64;  // z lanes per workgroup
 constexpr int kSub = 4;     // column groups per workgroup
-constexpr int kIB = 32;     // x cells per thread run
+constexpr int kIB = 16;     // x cells per thread run
 constexpr int kJB = 16;     // y rows per workgroup (kJB / kSub per thread)
 constexpr int kNQ1 = 6;     // pass-1 sums: ucc, vcc, wcc, u node, v node, w face
 constexpr int kNQ2 = 4;     // pass-2 sums: u'u' node, v'v' node, w'w' face, u'w' centre
@@ -47,14 +47,16 @@ struct Dims {
     int nx, ny, nz, nbx, nby;
 };
 
-// Plane sums over one workgroup's columns for z lane k (0..nz; k == nz is
-// the top w face only).  mean: pass-1 means, (6, nz + 1).
+// Plane sums over one workgroup's columns for z lane k (0..nz-1); lane
+// nz-1 also sums the top w face (index nz), which it loads anyway.  mean:
+// pass-1 means, (6, nz + 1).
 template <int PASS>
 __global__ __launch_bounds__(kLanes * kSub) void stats_pass_kernel(
     Dims d, const double* __restrict__ u, const double* __restrict__ v,
     const double* __restrict__ w, const double* __restrict__ mean, double* __restrict__ part) {
     constexpr int NQ = PASS == 1 ? kNQ1 : kNQ2;
     __shared__ double red[kSub][NQ][kLanes];
+    __shared__ double red_top[kSub];
     const int lane = threadIdx.x % kLanes, sub = threadIdx.x / kLanes;
     const int k = blockIdx.x * kLanes + lane;
     const int nx = d.nx, ny = d.ny, nz = d.nz, nzf = nz + 1;
@@ -65,15 +67,15 @@ __global__ __launch_bounds__(kLanes * kSub) void stats_pass_kernel(
     double s[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) s[q] = 0.0;
-    double mun = 0.0, mvn = 0.0, mwf = 0.0, mU = 0.0, mW = 0.0;
-    if (PASS == 2 && k <= nz) {
+    double top = 0.0;  // lane nz-1: sums of the top face
+    double mun = 0.0, mvn = 0.0, mwf = 0.0, mU = 0.0, mW = 0.0, mwt = 0.0;
+    if (PASS == 2 && k < nz) {
         mwf = mean[5 * nzf + k];
-        if (k < nz) {
-            mU = mean[0 * nzf + k];
-            mW = mean[2 * nzf + k];
-            mun = mean[3 * nzf + k];
-            mvn = mean[4 * nzf + k];
-        }
+        mU = mean[0 * nzf + k];
+        mW = mean[2 * nzf + k];
+        mun = mean[3 * nzf + k];
+        mvn = mean[4 * nzf + k];
+        if (k == nz - 1) mwt = mean[5 * nzf + nz];
     }
     for (int j = j0 + sub; j < j1; j += kSub) {
         // u[i0, j+1, k+1], v[i0+1, j, k+1], w[i0+1, j+1, k]
@@ -103,24 +105,23 @@ __global__ __launch_bounds__(kLanes * kSub) void stats_pass_kernel(
                     s[2] += dw * dw;
                     s[3] += (ucc - mU) * (wcc - mW);
                 }
-                up = un;
-            }
-        } else if (k == nz) {
-            for (int i = i0; i < i1; ++i) {
-                const double w0 = wr[(size_t)(i - i0) * sw];
-                if (PASS == 1) {
-                    s[5] += w0;
-                } else {
-                    const double dw = w0 - mwf;
-                    s[2] += dw * dw;
+                if (k == nz - 1) {
+                    if (PASS == 1) {
+                        top += w1;
+                    } else {
+                        const double dt = w1 - mwt;
+                        top += dt * dt;
+                    }
                 }
+                up = un;
             }
         }
     }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) red[sub][q][lane] = s[q];
+    if (k == nz - 1) red_top[sub] = top;
     __syncthreads();
-    if (sub == 0 && k <= nz) {
+    if (sub == 0 && k < nz) {
         const size_t chunk = (size_t)blockIdx.z * d.nbx + blockIdx.y;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
@@ -129,26 +130,37 @@ __global__ __launch_bounds__(kLanes * kSub) void stats_pass_kernel(
             for (int t = 1; t < kSub; ++t) r += red[t][q][lane];
             part[(chunk * NQ + q) * nzf + k] = r;
         }
+        if (k == nz - 1) {
+            double r = red_top[0];
+            for (int t = 1; t < kSub; ++t) r += red_top[t];
+            // the top face is quantity 5 (w face) in pass 1, 2 (w'w' face) in
+            // pass 2; the other quantities have no entry there (zero)
+            constexpr int QT = PASS == 1 ? 5 : 2;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) part[(chunk * NQ + q) * nzf + nz] = q == QT ? r : 0.0;
+        }
     }
 }
 
 // Sums the partials of quantity q = blockIdx.y at lane k in chunk order
-// (column group t takes chunks t, t + 4, ...; groups combined 0..3) and
-// divides by the plane size: out[q * (nz + 1) + k].
-__global__ __launch_bounds__(kLanes * kSub) void stats_reduce_kernel(
+// (column group t of kRSub takes chunks t, t + kRSub, ...; groups combined in
+// order) and divides by the plane size: out[q * (nz + 1) + k].
+constexpr int kRSub = 16;
+
+__global__ __launch_bounds__(kLanes * kRSub) void stats_reduce_kernel(
     const double* __restrict__ part, int nchunks, int NQ, int nz, double plane,
     double* __restrict__ out) {
-    __shared__ double red[kSub][kLanes];
+    __shared__ double red[kRSub][kLanes];
     const int lane = threadIdx.x % kLanes, sub = threadIdx.x / kLanes;
     const int k = blockIdx.x * kLanes + lane, q = blockIdx.y, nzf = nz + 1;
     double r = 0.0;
     if (k <= nz)
-        for (int c = sub; c < nchunks; c += kSub) r += part[((size_t)c * NQ + q) * nzf + k];
+        for (int c = sub; c < nchunks; c += kRSub) r += part[((size_t)c * NQ + q) * nzf + k];
     red[sub][lane] = r;
     __syncthreads();
     if (sub == 0 && k <= nz) {
         double t = red[0][lane];
-        for (int g = 1; g < kSub; ++g) t += red[g][lane];
+        for (int g = 1; g < kRSub; ++g) t += red[g][lane];
         out[(size_t)q * nzf + k] = t / plane;
     }
 }
@@ -240,7 +252,7 @@ __global__ __launch_bounds__(256) void stats_fold_kernel(Dims d, const double2* 
 struct slc_stats {
     int device = 0;
     Dims d{};
-    int nzc = 0, nchunks = 0, nplanes = 0;
+    int nzc = 0, nzr = 0, nchunks = 0, nplanes = 0;
     double* part = nullptr;    // nchunks x 6 x (nz + 1)
     double* mean = nullptr;    // 6 x (nz + 1)
     double* mom = nullptr;     // 4 x (nz + 1)
@@ -303,7 +315,8 @@ int slc_stats_create(int device, int nx, int ny, int nz, int nplanes, const int*
     slc_stats* st = new slc_stats();
     st->device = device;
     st->d = Dims{nx, ny, nz, (nx + kIB - 1) / kIB, (ny + kJB - 1) / kJB};
-    st->nzc = (nz + 1 + kLanes - 1) / kLanes;
+    st->nzc = (nz + kLanes - 1) / kLanes;        // pass workgroups along z
+    st->nzr = (nz + 1 + kLanes - 1) / kLanes;    // reduce workgroups along z (faces)
     st->nchunks = st->d.nbx * st->d.nby;
     st->nplanes = nplanes;
     const size_t nzf = (size_t)nz + 1;
@@ -380,10 +393,12 @@ int slc_stats_accumulate(slc_stats* st, const double* u, const double* v, const 
     const double plane = (double)d.nx * (double)d.ny;
     hipLaunchKernelGGL(stats_pass_kernel<1>, grid, blk, 0, s, d, u, v, w, st->mean, st->part);
     HIP_OK(hipGetLastError());
-    hipLaunchKernelGGL(stats_reduce_kernel, dim3(st->nzc, kNQ1), blk, 0, s, st->part, st->nchunks,
+    hipLaunchKernelGGL(stats_reduce_kernel, dim3(st->nzr, kNQ1), dim3(kLanes * kRSub), 0, s,
+                       st->part, st->nchunks,
                        kNQ1, d.nz, plane, st->mean);
     hipLaunchKernelGGL(stats_pass_kernel<2>, grid, blk, 0, s, d, u, v, w, st->mean, st->part);
-    hipLaunchKernelGGL(stats_reduce_kernel, dim3(st->nzc, kNQ2), blk, 0, s, st->part, st->nchunks,
+    hipLaunchKernelGGL(stats_reduce_kernel, dim3(st->nzr, kNQ2), dim3(kLanes * kRSub), 0, s,
+                       st->part, st->nchunks,
                        kNQ2, d.nz, plane, st->mom);
     hipLaunchKernelGGL(stats_update_kernel, dim3((d.nz + 255) / 256), dim3(256), 0, s, d.nz,
                        st->mean, st->mom, prof);
