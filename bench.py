@@ -557,6 +557,7 @@ def ensemble_bench(args):
         "autotune_ms": {"tile_shape": tune, "grid_per_cu": tune_grid},
         "member0_identical_to_plain": bool(same0),
         "fp32_storage_rel_err_vs_fp64": {k: float(f"{v:.3e}") for k, v in err.items()},
+        "fp32_storage_err_after_steps": W + K,
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)

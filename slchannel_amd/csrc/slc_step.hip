@@ -17,13 +17,15 @@ namespace slc {
 #endif
 
 #ifndef SLC_TILE_WAVES
-#define SLC_TILE_ATTR
-#else
-#define SLC_TILE_ATTR __attribute__((amdgpu_waves_per_eu(SLC_TILE_WAVES)))
+#define SLC_TILE_WAVES 4
+#endif
+// waves per SIMD the near pass and the gather at given feet are built for
+#ifndef SLC_TILE_WAVES_GATHER
+#define SLC_TILE_WAVES_GATHER SLC_TILE_WAVES
 #endif
 
-template <typename T, int C, int P, int MODE, int TI, int TJ, int TK, int LDS_BYTES>
-__global__ __launch_bounds__(kSB) SLC_TILE_ATTR void sl_tile_kernel(
+template <typename T, int C, int P, int MODE, int TI, int TJ, int TK, int LDS_BYTES, int WAVES>
+__global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(WAVES))) void sl_tile_kernel(
     Grid g, ArrivalTabs A, TrajVel V, const double* __restrict__ F,
     const double* __restrict__ Fm, const T* __restrict__ Sn, const T* __restrict__ Sm,
     double dt, double* __restrict__ out, Feet ft, unsigned long long* nclamp) {
@@ -190,7 +192,9 @@ static void launch_mode(const Grid& g, const ArrivalTabs& A, const TrajVel& V,
     const unsigned tiles = (unsigned)(((long)(g.nx + kTI - 1) / kTI) *
                                       ((g.ny + kTJ - 1) / kTJ) * ((nzi + kTK - 1) / kTK));
     if (tiles == 0) return;
-    hipLaunchKernelGGL((sl_tile_kernel<T, C, P, MODE, kTI, kTJ, kTK, kLDS>), dim3(tiles),
+    constexpr int W = (MODE == MODE_NEAR || MODE == MODE_GATHER) ? SLC_TILE_WAVES_GATHER
+                                                                  : SLC_TILE_WAVES;
+    hipLaunchKernelGGL((sl_tile_kernel<T, C, P, MODE, kTI, kTJ, kTK, kLDS, W>), dim3(tiles),
                        dim3(kSB), 0, s, g, A, V, a.F, a.Fm, (const T*)a.Sn, (const T*)a.Sm, a.dt,
                        a.out, a.ft, a.ncl);
 }
