@@ -61,8 +61,14 @@ def parse():
                          "step at the reference's Eulerian CFL 0.28, with both schemes' "
                          "simulated time per second")
     ap.add_argument("--members", type=int, default=48, help="config 4: ensemble size")
+    ap.add_argument("--ensemble-grids", default="survey", choices=("survey", "small"),
+                    help="config 4: member grids -- the workload's four sizes, or small "
+                         "launch-bound members (32x24x32, 48x32x48, 32x32x24)")
     ap.add_argument("--requests", type=int, default=24, help="config 5: number of requests")
     ap.add_argument("--rate", type=float, default=1.0, help="config 5: arrivals per second")
+    ap.add_argument("--serve-grids", default="survey", choices=("survey", "small"),
+                    help="config 5: request grids -- the workload's four sizes, or small "
+                         "launch-bound ones (32x24x32, 48x32x48, 32x32x24)")
     ap.add_argument("--hbm-budget-gb", type=float, default=48.0,
                     help="config 5: HBM budget of the admission control")
     return ap.parse_args()
@@ -443,6 +449,7 @@ def large_box(args):
 
 
 ENSEMBLE_GRIDS = ((64, 48, 64), (96, 64, 96), (128, 96, 128), (192, 128, 128))
+SMALL_GRIDS = ((32, 24, 32), (48, 32, 48), (32, 32, 24))
 
 
 def _member_cfg(nx, ny, nz, Re, order, fp32=False):
@@ -468,10 +475,11 @@ def ensemble_bench(args):
     from slchannel_amd.diagnostics import cfl_dt
     from slchannel_amd.ensemble import OPT_GRAPH, Ensemble
 
+    grids = ENSEMBLE_GRIDS if args.ensemble_grids == "survey" else SMALL_GRIDS
     rng = np.random.Generator(np.random.PCG64(4000))
     specs = []
     for m in range(args.members):
-        nx, ny, nz = ENSEMBLE_GRIDS[int(rng.integers(len(ENSEMBLE_GRIDS)))]
+        nx, ny, nz = grids[int(rng.integers(len(grids)))]
         specs.append((nx, ny, nz, float(rng.uniform(2800.0, 6000.0)), 4000 + m))
     inits, flows, dts = [], [], []
     for nx, ny, nz, Re, seed in specs:
@@ -541,12 +549,12 @@ def ensemble_bench(args):
         for k in "uvwp":
             err[k] = max(err[k], (fb[k] - fa[k]).abs().max().item() / fa[k].abs().max().item())
     line = {
-        "metric": "member-steps/s (48-member uneven SL-BDF2 ensemble, triquintic fp64)",
+        "metric": f"member-steps/s ({args.members}-member uneven SL-BDF2 ensemble, triquintic fp64)",
         "value": round(batched, 2), "unit": "member-steps/s", "n_gpus": 1, "steps": K,
         "warmup": W, "ms_per_step": round(min(eager_s, graph_s) * 1e3 / K, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded solenoidal perturbations on Poiseuille, projected)",
-        "config": {"workload": f"{args.members} members from {list(ENSEMBLE_GRIDS)}, "
+        "config": {"workload": f"{args.members} members from {list(grids)}, "
                                "Re U[2800,6000], order 6, CFL 3",
                    "members": args.members, "grid_points": points},
         "plain_per_run_member_steps_per_s": round(plain, 2),
@@ -575,6 +583,8 @@ def serve_bench(args):
     from slchannel_amd.serve import Server, plain_fifo, poisson_requests, run_alone
 
     grids = ((64, 64, 64), (128, 96, 128), (256, 256, 256), (384, 384, 256))
+    if args.serve_grids == "small":
+        grids = SMALL_GRIDS
     reqs = poisson_requests(args.requests, args.rate, grids, (2800.0, 10000.0), (20, 200),
                             seed=5000, order=args.order)
     srv = Server(hbm_budget=int(args.hbm_budget_gb * 1e9))

@@ -148,9 +148,11 @@ def _solenoidal_torch(nx, ny, nz, Lx, Lz, z_f, z_c, urms, U_bulk, chunk, device,
     up = evaluate(fu, zc_in, dx, 0.5 * dy)
     vp = evaluate(fv, zc_in, 0.5 * dx, dy)
     wp = evaluate(fw, zf_in, 0.5 * dx, 0.5 * dy)
-    ss = ((up**2).sum() + (vp**2).sum() + (wp**2).sum()).item() / (up.numel() + vp.numel()
-                                                                   + wp.numel())
-    scale = urms / math.sqrt(ss) if ss > 0 else 0.0
+    # the scale stays on the device (no host synchronisation): the same IEEE
+    # operations as urms / math.sqrt(sum / count) on the host
+    ss = ((up**2).sum() + (vp**2).sum() + (wp**2).sum()) / (up.numel() + vp.numel()
+                                                            + wp.numel())
+    scale = torch.where(ss > 0, urms / torch.sqrt(ss), torch.zeros_like(ss))
     f64 = dict(dtype=torch.float64, device=dev)
     u = torch.zeros((nx + 1, ny + 2, nz + 2), **f64)
     v = torch.zeros((nx + 2, ny + 1, nz + 2), **f64)

@@ -36,8 +36,8 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from .diagnostics import BlowupGuard, cfl_dt
-from .ensemble import Ensemble, _INIT_CACHE, _config_grid, initial_fields, member_bytes
+from .diagnostics import BlowupGuard
+from .ensemble import N_STATS, Ensemble, _INIT_CACHE, _config_grid, initial_fields, member_bytes
 
 
 @dataclass
@@ -70,24 +70,50 @@ class Result:
     status: str = "pending"   # ok | rejected_nonfinite | stopped_nonfinite | stopped_blowup
 
 
-def _prepare(req, device):
-    """Initial state and dt of a request on the device (SLChannelFlow.initialize,
-    or the request's own fields)."""
+def _grid_key(config, device=None):
+    c = _config_grid(config)
+    key = (c["nx"], c["ny"], c["nz"], c["Lx"], c["Ly"], c["Lz"], c["gamma"], c["stretching"],
+           c["top"], c["order"])
+    return key if device is None else key + (str(device),)
+
+
+def _prepare_async(req, device):
+    """Initial state of a request on the device (SLChannelFlow.initialize, or
+    the request's own fields), launched on the current stream without a host
+    synchronisation.  Returns (fields, dt, probe): dt is the request's own dt
+    or None; probe is a 2-double device tensor [dti, number of non-finite
+    entries] (dti: the trajectory CFL inverse time-scale, operators.py:839-940,
+    via slc_diagnostics) for the caller to read once the stream is done."""
+    probe = torch.zeros(2, dtype=torch.float64, device=device)
     if req.fields is not None:
         u, v, w = (t.to(device=device, dtype=torch.float64).contiguous() for t in req.fields[:3])
         p = req.fields[3] if len(req.fields) > 3 else None
         if req.dt is None:
             raise ValueError(f"request {req.rid}: explicit fields need an explicit dt")
-        return (u, v, w, p), float(req.dt)
+        bad = sum((~torch.isfinite(t)).sum() for t in (u, v, w, p) if t is not None)
+        probe[1:2].copy_(bad.to(torch.float64).reshape(1))
+        return (u, v, w, p), float(req.dt), probe
     u, v, w, p = initial_fields(req.config, req.seed, urms=req.urms, kmax=req.kmax,
                                 device=device)
-    dt = req.dt
+    if req.dt is not None:
+        return (u, v, w, p), float(req.dt), probe
+    out = torch.empty(5, dtype=torch.float64, device=device)
+    _INIT_CACHE[_grid_key(req.config, device)].plan.diagnostics(u, v, w, None, 0.0, out)
+    probe[0:1].copy_(out[3:4])
+    return (u, v, w, p), None, probe
+
+
+def _dt_of(req, dti):
+    """cfl_dt's rule: dt = CFL / dti (dti > 1e-10, else CFL)."""
+    return req.cfl / (dti if dti > 1e-10 else 1.0)
+
+
+def _prepare(req, device):
+    """_prepare_async plus the one host read of dt."""
+    fields, dt, probe = _prepare_async(req, device)
     if dt is None:
-        c = _config_grid(req.config)
-        key = (c["nx"], c["ny"], c["nz"], c["Lx"], c["Ly"], c["Lz"], c["gamma"], c["stretching"],
-               c["top"], c["order"], str(device))
-        dt = cfl_dt(_INIT_CACHE[key], req.cfl)
-    return (u, v, w, p), float(dt)
+        dt = _dt_of(req, float(probe[0].item()))
+    return fields, float(dt)
 
 
 class Server:
@@ -106,9 +132,18 @@ class Server:
         self.storage = storage
         self.guard_every = int(guard_every)
         self.blowup_factor = float(blowup_factor)
-        # admissions (initial fields, dt) run on their own stream so they
-        # overlap the batched steps instead of queueing behind them
-        self.side = torch.cuda.Stream(self.device)
+        # admissions (initial fields, dt) run on one stream per grid size, so
+        # they overlap the batched steps instead of queueing behind them, and
+        # two admissions of one grid (which share its initialiser's plan)
+        # stay ordered
+        self._streams = {}
+
+    def _stream_for(self, config):
+        key = _grid_key(config)
+        st = self._streams.get(key)
+        if st is None:
+            st = self._streams[key] = torch.cuda.Stream(self.device)
+        return st
 
     def warmup(self, configs):
         """Server start-up for a grid catalogue: builds the per-grid initialiser
@@ -121,13 +156,14 @@ class Server:
             if key in seen:
                 continue
             seen.add(key)
-            with torch.cuda.stream(self.side):
+            with torch.cuda.stream(self._stream_for(cfg)):
                 initial_fields(cfg, 0, device=self.device)
         torch.cuda.synchronize(self.device)
 
-    def fits(self, req):
+    def fits(self, req, reserved=0, preparing=0):
         need = member_bytes(req.config, self.storage)
-        return need <= self.budget - self.ens.bytes() and len(self.ens) < self.max_members
+        return (need <= self.budget - self.ens.bytes() - reserved
+                and len(self.ens) + preparing < self.max_members)
 
     def run(self, requests, log=None, inflight=3):
         """Serve `requests` (arrival times relative to the start); returns
@@ -142,7 +178,8 @@ class Server:
         results = {}
         active = {}  # member id -> (request, result)
         queued = []  # (start event, end event, results stepped)
-        retiring = []  # (event after the copies out, request, result)
+        retiring = []  # (event after the copies out, request, result, pinned stats, diag)
+        preparing = []  # (event, request, fields, dt, pinned probe, stream, bytes)
         guards = {}  # member id -> BlowupGuard
         checks = []  # (event, member id, rid, pinned diagnostics)
         t0 = time.perf_counter()
@@ -161,24 +198,38 @@ class Server:
                 for res in stepped:
                     res.latencies_ms.append(ms)
 
-        while pending or active:
+        while pending or active or preparing:
             now = time.perf_counter() - t0
-            # admission: FIFO over the requests that have arrived, one per
-            # iteration so the device always has a queued step meanwhile
-            if pending and pending[0].arrival <= now and self.fits(pending[0]):
+            ta = time.perf_counter()
+            # admission, FIFO over the requests that have arrived: the initial
+            # state is launched on the grid's stream and read back (dt, the
+            # finiteness check) through pinned memory once that stream is done
+            reserved = sum(x[6] for x in preparing)
+            if pending and pending[0].arrival <= now and \
+                    self.fits(pending[0], reserved, len(preparing)):
                 req = pending.pop(0)
-                ta = time.perf_counter()
-                with torch.cuda.stream(self.side):
-                    fields, dt = _prepare(req, self.device)
-                    if req.fields is not None and not all(
-                            bool(torch.isfinite(t).all()) for t in fields if t is not None):
-                        res = Result(rid=req.rid, arrival=req.arrival, dt=dt,
-                                     status="rejected_nonfinite")
-                        res.admitted = res.finished = time.perf_counter() - t0
-                        results[req.rid] = res
-                        continue
+                st = self._stream_for(req.config)
+                with torch.cuda.stream(st):
+                    fields, dt, probe = _prepare_async(req, self.device)
+                    h = torch.empty(2, dtype=torch.float64, pin_memory=True)
+                    h.copy_(probe, non_blocking=True)
+                    ev = torch.cuda.Event()
+                    ev.record()
+                preparing.append((ev, req, fields, dt, h, st,
+                                  member_bytes(req.config, self.storage)))
+            while preparing and preparing[0][0].query():
+                _, req, fields, dt, h, st, _ = preparing.pop(0)
+                probe = h.tolist()
+                if dt is None:
+                    dt = _dt_of(req, probe[0])
+                if probe[1] > 0:
+                    res = Result(rid=req.rid, arrival=req.arrival, dt=dt,
+                                 status="rejected_nonfinite")
+                    res.admitted = res.finished = time.perf_counter() - t0
+                    results[req.rid] = res
+                    continue
+                with torch.cuda.stream(st):
                     mid = self.ens.add(req.config, dt, fields=fields)
-                host["admit_s"] += time.perf_counter() - ta
                 res = Result(rid=req.rid, admitted=time.perf_counter() - t0, dt=dt)
                 res.arrival = req.arrival
                 guards[mid] = BlowupGuard(req.u_tau_nominal, self.blowup_factor)
@@ -188,8 +239,11 @@ class Server:
                     c = _config_grid(req.config)
                     log(f"admit {req.rid} {c['nx']}x{c['ny']}x{c['nz']} "
                         f"steps={req.n_steps} active={len(active)}")
+            host["admit_s"] += time.perf_counter() - ta
             if not active:
-                if pending:
+                if preparing:
+                    time.sleep(1e-4)
+                elif pending:
                     wait = pending[0].arrival - (time.perf_counter() - t0)
                     if wait > 0:
                         time.sleep(min(wait, 0.05))
@@ -234,24 +288,24 @@ class Server:
                 # the arena is reused only by work queued after them
                 if self.keep_fields:
                     res.fields = self.ens.get_fields(mid)
-                res.stats = self.ens.stats(mid)
-                res.diag = self.ens.diagnostics(mid)
+                hs = torch.empty(N_STATS, dtype=torch.float64, pin_memory=True)
+                hd = torch.empty(5, dtype=torch.float64, pin_memory=True)
+                hs.copy_(self.ens.stats(mid), non_blocking=True)
+                hd.copy_(self.ens.diagnostics(mid), non_blocking=True)
                 ev = torch.cuda.Event()
                 ev.record()
-                retiring.append((ev, req, res))
+                retiring.append((ev, req, res, hs, hd))
                 self.ens.remove(mid)
             while retiring and retiring[0][0].query():
-                ev, req, res = retiring.pop(0)
+                ev, req, res, hs, hd = retiring.pop(0)
                 res.finished = time.perf_counter() - t0
-                res.stats = res.stats.cpu().tolist()
-                res.diag = res.diag.cpu().tolist()
+                res.stats, res.diag = hs.tolist(), hd.tolist()
                 results[req.rid] = res
             host["retire_s"] += time.perf_counter() - tr
-        for ev, req, res in retiring:
+        for ev, req, res, hs, hd in retiring:
             ev.synchronize()
             res.finished = time.perf_counter() - t0
-            res.stats = res.stats.cpu().tolist()
-            res.diag = res.diag.cpu().tolist()
+            res.stats, res.diag = hs.tolist(), hd.tolist()
             results[req.rid] = res
         drain(0)
         t_end = time.perf_counter() - t0
