@@ -124,8 +124,15 @@ def _solenoidal_torch(nx, ny, nz, Lx, Lz, z_f, z_c, urms, U_bulk, chunk, device,
     import torch
 
     dev = torch.device(device)
-    mi = torch.as_tensor(mm % nx, device=dev)
-    ni = torch.as_tensor(nn % ny, device=dev)
+
+    def upload(a):
+        # host arrays go through pinned memory with an asynchronous copy, so
+        # building an initial state never waits for work queued on the stream
+        t = torch.from_numpy(np.ascontiguousarray(a))
+        return t.pin_memory().to(dev, non_blocking=True)
+
+    mi = upload(mm % nx)
+    ni = upload(nn % ny)
 
     def evaluate(coef_fn, zs, xshift, yshift):
         out = torch.empty((nx, ny, len(zs)), dtype=torch.float64, device=dev)
@@ -133,7 +140,7 @@ def _solenoidal_torch(nx, ny, nz, Lx, Lz, z_f, z_c, urms, U_bulk, chunk, device,
         for z0 in range(0, len(zs), chunk):
             zz = zs[z0 : z0 + chunk]
             g, dg = G(zz)
-            coef = torch.as_tensor(coef_fn(g, dg) * phase[:, None], device=dev)
+            coef = upload(coef_fn(g, dg) * phase[:, None])
             C = torch.zeros((nx, ny, len(zz)), dtype=torch.complex128, device=dev)
             C.index_put_((mi, ni), coef, accumulate=True)
             out[:, :, z0 : z0 + len(zz)] = torch.fft.ifft2(C, dim=(0, 1)).real * (nx * ny)
@@ -158,7 +165,7 @@ def _solenoidal_torch(nx, ny, nz, Lx, Lz, z_f, z_c, urms, U_bulk, chunk, device,
     v = torch.zeros((nx + 2, ny + 1, nz + 2), **f64)
     w = torch.zeros((nx + 2, ny + 2, nz + 1), **f64)
     zeta = 2.0 * zc_in / Lz - 1.0
-    base = torch.as_tensor(1.5 * U_bulk * (1.0 - zeta**2), **f64)
+    base = upload(1.5 * U_bulk * (1.0 - zeta**2))
     u[1 : nx + 1, 1 : ny + 1, 1 : nz + 1] = scale * up + base[None, None, :]
     v[1 : nx + 1, 1 : ny + 1, 1 : nz + 1] = scale * vp
     w[1 : nx + 1, 1 : ny + 1, 1:nz] = scale * wp
