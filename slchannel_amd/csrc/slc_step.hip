@@ -23,6 +23,13 @@ namespace slc {
 #ifndef SLC_TILE_WAVES_GATHER
 #define SLC_TILE_WAVES_GATHER SLC_TILE_WAVES
 #endif
+// tile of the near pass and the gather at given feet (default: the same)
+#ifndef SLC_GTILE_TI
+#define SLC_GTILE_TI SLC_TILE_TI
+#define SLC_GTILE_TJ SLC_TILE_TJ
+#define SLC_GTILE_TK SLC_TILE_TK
+#define SLC_GTILE_LDS SLC_TILE_LDS
+#endif
 
 template <typename T, int C, int P, int MODE, int TI, int TJ, int TK, int LDS_BYTES, int WAVES>
 __global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(WAVES))) void sl_tile_kernel(
@@ -163,13 +170,15 @@ struct StepArgs {
 template <typename T, int C, int P, int MODE>
 static void launch_mode(const Grid& g, const ArrivalTabs& A, const TrajVel& V,
                         const StepArgs& a, hipStream_t s) {
+    constexpr bool G = MODE == MODE_NEAR || MODE == MODE_GATHER;
+    constexpr int TI = G ? SLC_GTILE_TI : kTI, TJ = G ? SLC_GTILE_TJ : kTJ;
+    constexpr int TK = G ? SLC_GTILE_TK : kTK, LDS = G ? SLC_GTILE_LDS : kLDS;
+    constexpr int W = G ? SLC_TILE_WAVES_GATHER : SLC_TILE_WAVES;
     const long nzi = (C == 2) ? g.nz - 1 : g.nz;
-    const unsigned tiles = (unsigned)(((long)(g.nx + kTI - 1) / kTI) *
-                                      ((g.ny + kTJ - 1) / kTJ) * ((nzi + kTK - 1) / kTK));
+    const unsigned tiles = (unsigned)(((long)(g.nx + TI - 1) / TI) *
+                                      ((g.ny + TJ - 1) / TJ) * ((nzi + TK - 1) / TK));
     if (tiles == 0) return;
-    constexpr int W = (MODE == MODE_NEAR || MODE == MODE_GATHER) ? SLC_TILE_WAVES_GATHER
-                                                                  : SLC_TILE_WAVES;
-    hipLaunchKernelGGL((sl_tile_kernel<T, C, P, MODE, kTI, kTJ, kTK, kLDS, W>), dim3(tiles),
+    hipLaunchKernelGGL((sl_tile_kernel<T, C, P, MODE, TI, TJ, TK, LDS, W>), dim3(tiles),
                        dim3(kSB), 0, s, g, A, V, a.F, a.Fm, (const T*)a.Sn, (const T*)a.Sm, a.dt,
                        a.out, a.ft, a.ncl);
 }
