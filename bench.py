@@ -50,6 +50,9 @@ def parse():
                     help="bits of the gathered field copies: 64 (default) or 32 "
                          "(FP32-storage variant; adds an FP64 twin run and reports the "
                          "per-field difference)")
+    ap.add_argument("--bdf2-split", action="store_true",
+                    help="far pass, then near pass, instead of the fused BDF2 pass "
+                         "(same doubles; for profiling the two passes)")
     ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5, 6),
                     help="2: 256^3 channel (default, the headline workload); "
                          "3: 1024x1024x512 large box, CFL 4, plus the isolated gather "
@@ -110,6 +113,8 @@ def main():
     }
     flow = SLChannelFlow(config=cfg, device="cuda:0")
     flow.initialize(seed=2000, urms=0.15, kmin=1.0, kmax=32.0, spectrum="k-5/3")
+    if args.bdf2_split:
+        flow.plan.set_option(flow.plan.OPT_BDF2_FUSED, 0)
     nu = flow.nu
     u_tau = Re_tau * nu  # delta = 1
     dt = dtp * nu / (u_tau * u_tau)
@@ -138,10 +143,7 @@ def main():
     ms_step = (t1 - t0) * 1e3 / args.steps
 
     N_u = n * n * nz
-    lab = "bdf2_far_u"
-    cnt, tot = prof[lab]
-    avg_ms = tot / cnt
-    bytes_far = (64.0 if args.storage == 64 else 60.0) * N_u
+    lab, kname, avg_ms, bytes_far = roofline_kernel(prof, N_u, args.storage)
     achieved = bytes_far / (avg_ms * 1e-3) / 1e9
 
     # isolated gather of u at the 2dt foot of the current state
@@ -172,7 +174,7 @@ def main():
                                     ("w", flow.w, twin.w), ("p", flow.p, twin.p))}
         fp32_err["steps"] = args.warmup + args.steps
 
-    traffic = measured_traffic(n, nz, args.order) if args.storage == 64 else None
+    traffic = measured_traffic(lab, n, nz, args.order) if args.storage == 64 else None
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(flow, dt)
@@ -196,8 +198,7 @@ def main():
                                f"Re_b {Re_b} dt+ {dtp}",
                    "nx": n, "ny": n, "nz": nz, "interp_order": args.order, "dt": dt,
                    "storage_bits": args.storage},
-        "roofline": {"bound": "hbm", "kernel": "sl_tile_kernel<u,P,FAR> (bdf2_far_u)",
-                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": traffic,
                      "bytes_per_launch": bytes_far, "avg_ms": round(avg_ms, 4)},
         "gather_GBps": round(gather_gbps, 1),
@@ -400,9 +401,8 @@ def large_box(args):
     prof = plan.profile_report()
     plan.profile(False)
     N_u = nx * ny * nz
-    cnt, tot = prof["bdf2_far_u"]
-    avg_ms = tot / cnt
-    achieved = 64.0 * N_u / (avg_ms * 1e-3) / 1e9
+    _, kname, avg_ms, bytes_k = roofline_kernel(prof, N_u, 64)
+    achieved = bytes_k / (avg_ms * 1e-3) / 1e9
     # isolated gather, feet = arrival + U(-5, 5) cells in x and y
     g = torch.Generator(device="cuda").manual_seed(33)
     ii = torch.arange(N_u, device="cuda", dtype=torch.int64)
@@ -434,10 +434,10 @@ def large_box(args):
         "data": "synthetic (seeded solenoidal k^-5/3 perturbation on Poiseuille, projected)",
         "config": {"workload": "SL-BDF2 channel 1024x1024x512 order 6 Re_b 20000 CFL 4",
                    "nx": nx, "ny": ny, "nz": nz, "interp_order": args.order, "dt": dt},
-        "roofline": {"bound": "hbm", "kernel": "sl_tile_kernel<u,P,FAR> (bdf2_far_u)",
-                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
+                     "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
-                     "bytes_per_launch": 64.0 * N_u, "avg_ms": round(avg_ms, 4)},
+                     "bytes_per_launch": bytes_k, "avg_ms": round(avg_ms, 4)},
         "gather_random_pm5_GBps": round(gather_gbps, 1),
         "max_div": div, "u_bulk": ub.item(),
         "kernels_ms": {k: {"launches": c, "avg_ms": round(t / c, 4)}
@@ -625,11 +625,32 @@ def serve_bench(args):
     print(json.dumps(line), flush=True)
 
 
-def measured_traffic(n, nz, order):
-    """HBM bytes per launch of the roofline kernel from the committed PMC
-    summary (profiles/traffic_*.json, FETCH_SIZE + WRITE_SIZE scaled by the
-    copy-kernel calibration), if one exists for this workload; else None."""
-    path = os.path.join(REPO, "profiles", f"traffic_far_u_{n}x{n}x{nz}_o{order}.json")
+def roofline_kernel(prof, n_pts, storage):
+    """The dominant kernel of the SL-BDF2 step: the fused BDF2 pass of u
+    (default), else the far pass of u.  Returns (label, kernel name, average
+    launch ms, algorithmic bytes per launch).  Algorithmic bytes per arrival
+    point, each array counted once: fused -- U* 3x8 + V^{n-1} 8 + V^n 8 read
+    (each gathered and differenced), the prediction 8 written (float storage:
+    + 4 + 4 for the gathered float copies); far -- U* 3x8 + V^{n-1} 8 read,
+    the far value 8 + the near foot 3x8 written (float storage: the gathered
+    field 4 instead of 8)."""
+    fused = "bdf2_u" in prof
+    lab = "bdf2_u" if fused else "bdf2_far_u"
+    cnt, tot = prof[lab]
+    if fused:
+        per_pt = 48.0 if storage == 64 else 56.0
+    else:
+        per_pt = 64.0 if storage == 64 else 60.0
+    kname = "sl_tile_kernel<u,P,%s> (%s)" % ("BDF2" if fused else "FAR", lab)
+    return lab, kname, tot / cnt, per_pt * n_pts
+
+
+def measured_traffic(label, n, nz, order):
+    """HBM bytes per launch of the roofline kernel (launch label ``label``)
+    from the committed PMC summary (profiles/traffic_<label>_*.json,
+    FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration), if one
+    exists for this workload; else None."""
+    path = os.path.join(REPO, "profiles", f"traffic_{label}_{n}x{n}x{nz}_o{order}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:

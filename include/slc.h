@@ -218,9 +218,15 @@ int slc_bulk_force(slc_plan* plan, double* u, double* v, double* w, double U_bul
  * interp_dtype (semilag.py:340-356, semilag_triton.py:320-342), whose Triton
  * gather stores and accumulates in fp32.  Selecting 32 allocates the float
  * copies (2 x 4 B per field node) once.  Applies when
- * SLC_OPT_GATHER_TILE >= 0; slc_gather always reads FP64. */
+ * SLC_OPT_GATHER_TILE >= 0; slc_gather always reads FP64.
+ * SLC_OPT_BDF2_FUSED (no counterpart in the reference): 1 (default) = one
+ * LDS-tiled pass per component forms both departure feet and both gathers
+ * of slc_bdf2_predict; 0 = a far pass, then a near pass reading the near
+ * feet and the far values back from device memory.  Both give the same
+ * doubles; applies when SLC_OPT_GATHER_TILE >= 0. */
 #define SLC_OPT_GATHER_TILE 1
 #define SLC_OPT_FIELD_STORAGE 2
+#define SLC_OPT_BDF2_FUSED 3
 int slc_set_option(slc_plan* plan, int option, int value);
 
 /* Opt-in launch timing (no counterpart in the reference, whose benchmarks

@@ -109,6 +109,7 @@ const char* kLabel[3][6] = {
     {"bdf2_far_w", "bdf2_near_w", "bdf1_w", "advect_w", "gather_w", "departure_w"},
 };
 const char* kZLabel[3] = {"zsolve_u", "zsolve_v", "zsolve_w"};
+const char* kBdf2Label[3] = {"bdf2_u", "bdf2_v", "bdf2_w"};
 
 int fft_exec(slc_plan* p, bool forward, void* stream) {
     FFT_OK(rocfft_execution_info_set_stream(p->info, stream));
@@ -508,6 +509,15 @@ int slc_bdf2_predict(slc_plan* p, const double* u, const double* v, const double
         }
         TrajVel V{{p->us[0], p->us[1], p->us[2]}, {nullptr, nullptr, nullptr}};
         const long N = (long)p->nx * p->ny * p->nz;
+        if (p->bdf2_fused) {
+            for (int c = 0; c < 3; ++c) {
+                Timed t(p, kBdf2Label[c], stream);
+                HIP_OK(run_tile_step(p->g, c, 5, p->at[c], V, Fn[c], Fm[c],
+                                     f32 ? p->f32n[c] : nullptr, f32 ? p->f32m[c] : nullptr, dt,
+                                     O[c], nullptr, nullptr, nullptr, ncl, S(stream)));
+            }
+            return SLC_OK;
+        }
         for (int c = 0; c < 3; ++c)
             for (int pass = 0; pass < 2; ++pass) {
                 Timed t(p, kLabel[c][pass], stream);
@@ -625,6 +635,10 @@ int slc_set_option(slc_plan* p, int option, int value) {
         case SLC_OPT_GATHER_TILE:
             if (value < -1 || value > 3) return fail(SLC_EINVAL, "gather tile shape %d", value);
             p->gather_shape = value;
+            return SLC_OK;
+        case SLC_OPT_BDF2_FUSED:
+            if (value != 0 && value != 1) return fail(SLC_EINVAL, "bdf2 fused must be 0 or 1");
+            p->bdf2_fused = value;
             return SLC_OK;
         case SLC_OPT_FIELD_STORAGE: {
             if (value != 32 && value != 64)

@@ -47,7 +47,7 @@ struct ArrivalTabs {
 };
 
 // LDS-tiled step kernels (slc_step.hip); mode: 0 advect, 1 bdf1, 2 far,
-// 3 near, 4 gather (feet given)
+// 3 near, 4 gather (feet given), 5 far + near in one pass (no feet arrays)
 // Fn32 / Fm32: float copies of F / Fm to gather from (FP32 storage), or null.
 hipError_t run_tile_step(const Grid& g, int comp, int mode, const ArrivalTabs& A,
                          const TrajVel& V, const double* F, const double* Fm, const float* Fn32,

@@ -75,6 +75,7 @@ struct slc_plan {
     // opt-in launch timing: hipEvents recorded around each labelled launch
     bool prof = false;
     int gather_shape = 0;  // -1: direct global loads; >= 0: LDS tile shape
+    int bdf2_fused = 1;    // 1: one pass per component (MODE_BDF2); 0: far then near pass
     int storage = 64;      // 64: gathers read FP64 fields; 32: float copies (f32n / f32m)
     float* f32n[3] = {nullptr, nullptr, nullptr};  // float copy of V^n per component
     float* f32m[3] = {nullptr, nullptr, nullptr};  // float copy of V^{n-1} per component

@@ -23,6 +23,10 @@ namespace slc {
 #ifndef SLC_TILE_WAVES_GATHER
 #define SLC_TILE_WAVES_GATHER SLC_TILE_WAVES
 #endif
+// waves per SIMD the fused far + near pass is built for
+#ifndef SLC_TILE_WAVES_BDF2
+#define SLC_TILE_WAVES_BDF2 SLC_TILE_WAVES
+#endif
 // tile of the near pass and the gather at given feet (default: the same)
 #ifndef SLC_GTILE_TI
 #define SLC_GTILE_TI SLC_TILE_TI
@@ -37,11 +41,12 @@ __global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     const double* __restrict__ Fm, const T* __restrict__ Sn, const T* __restrict__ Sm,
     double dt, double* __restrict__ out, Feet ft, unsigned long long* nclamp) {
     __shared__ T lds[LDS_BYTES / sizeof(T)];
-    __shared__ int sh[6];
+    __shared__ int sh[MODE == MODE_BDF2 ? 12 : 6];
     __shared__ int s_cnt;
+    __shared__ FootSt nst[MODE == MODE_BDF2 ? TI * TJ * TK : 1];  // near stencils (MODE_BDF2)
     sl_tile_body<T, C, P, MODE, TI, TJ, TK, LDS_BYTES>(g, A, V, F, Fm, Sn, Sm, dt, out, ft,
                                                        nclamp, xcd_block(gridDim.x), lds, sh,
-                                                       &s_cnt);
+                                                       &s_cnt, nst);
 }
 
 // Arrival tables of component C: x/dx, y/dy of the arrival columns and the
@@ -173,7 +178,8 @@ static void launch_mode(const Grid& g, const ArrivalTabs& A, const TrajVel& V,
     constexpr bool G = MODE == MODE_NEAR || MODE == MODE_GATHER;
     constexpr int TI = G ? SLC_GTILE_TI : kTI, TJ = G ? SLC_GTILE_TJ : kTJ;
     constexpr int TK = G ? SLC_GTILE_TK : kTK, LDS = G ? SLC_GTILE_LDS : kLDS;
-    constexpr int W = G ? SLC_TILE_WAVES_GATHER : SLC_TILE_WAVES;
+    constexpr int W = G ? SLC_TILE_WAVES_GATHER
+                        : (MODE == MODE_BDF2 ? SLC_TILE_WAVES_BDF2 : SLC_TILE_WAVES);
     const long nzi = (C == 2) ? g.nz - 1 : g.nz;
     const unsigned tiles = (unsigned)(((long)(g.nx + TI - 1) / TI) *
                                       ((g.ny + TJ - 1) / TJ) * ((nzi + TK - 1) / TK));
@@ -208,6 +214,7 @@ static void launch_t(const Grid& g, int comp, int mode, const ArrivalTabs& A, co
         case MODE_BDF1: launch_c<T, MODE_BDF1>(g, comp, A, V, a, s); break;
         case MODE_FAR: launch_c<T, MODE_FAR>(g, comp, A, V, a, s); break;
         case MODE_NEAR: launch_c<T, MODE_NEAR>(g, comp, A, V, a, s); break;
+        case MODE_BDF2: launch_c<T, MODE_BDF2>(g, comp, A, V, a, s); break;
         default: launch_c<T, MODE_GATHER>(g, comp, A, V, a, s); break;
     }
 }

@@ -15,6 +15,12 @@
 //   MODE_NEAR    V^n gathered at the stored near foot, then
 //                out = 4/3 near - 1/3 out + (2dt/3)(2 R^n - R^{n-1})
 //                                                       (solver.py:885-903)
+//   MODE_BDF2    MODE_FAR and MODE_NEAR in one pass: the near foot's stencil
+//                waits in LDS while the far box is staged and contracted,
+//                then the near box is staged into the same buffer; the far
+//                value stays in a register.  The same operations as the two
+//                passes, so the same doubles, without the near foot and the
+//                far value going through HBM.
 // A workgroup owns a TI x TJ x TK tile of arrival points (z fastest).  Phase 1
 // traces the tile's feet and keeps each foot's stencil (base indices,
 // fractional offsets, clamped z) in registers; the tile's stencil bounding
@@ -33,7 +39,8 @@ namespace slc {
 
 constexpr int kSB = 256;  // threads per workgroup
 
-enum { MODE_ADVECT = 0, MODE_BDF1 = 1, MODE_FAR = 2, MODE_NEAR = 3, MODE_GATHER = 4 };
+enum { MODE_ADVECT = 0, MODE_BDF1 = 1, MODE_FAR = 2, MODE_NEAR = 3, MODE_GATHER = 4,
+       MODE_BDF2 = 5 };
 
 // Foot coordinates of one component (flat arrival order): written by
 // MODE_FAR (near feet), read by MODE_NEAR and MODE_GATHER.
@@ -61,6 +68,26 @@ __device__ inline int s_wave_max(int v) {
         v = w > v ? w : v;
     }
     return v;
+}
+
+// Workgroup union of the lanes' boxes, left in sh[0..5] (xmin, xmax, ymin,
+// ymax, kmin, kmax); ends with a barrier.
+__device__ inline void s_box_reduce(SBox l, int* sh) {
+    if (threadIdx.x == 0) {
+        sh[0] = INT_MAX; sh[1] = INT_MIN;
+        sh[2] = INT_MAX; sh[3] = INT_MIN;
+        sh[4] = INT_MAX; sh[5] = INT_MIN;
+    }
+    __syncthreads();
+    int a = s_wave_min(l.xmin), b = s_wave_max(l.xmax);
+    int c = s_wave_min(l.ymin), d = s_wave_max(l.ymax);
+    int e = s_wave_min(l.kmin), f = s_wave_max(l.kmax);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&sh[0], a); atomicMax(&sh[1], b);
+        atomicMin(&sh[2], c); atomicMax(&sh[3], d);
+        atomicMin(&sh[4], e); atomicMax(&sh[5], f);
+    }
+    __syncthreads();
 }
 
 __device__ inline SBox s_block_box(SBox l, int* sh) {
@@ -244,13 +271,14 @@ __device__ inline void trace_foot(const Grid& g, const TrajA& V, double xa, doub
 // FP64).  F / Fm are always the FP64 fields (explicit xy-diffusion);
 // Sn / Sm are the gathered copies (== F / Fm when T is double).
 // One tile of the step kernel (tile = linear tile index, k fastest).  The
-// caller provides the block's LDS: lds (LDS_BYTES), sh (6 ints), s_cnt.
+// caller provides the block's LDS: lds (LDS_BYTES), sh (6 ints; 12 for
+// MODE_BDF2), s_cnt, and for MODE_BDF2 nst (one FootSt per tile point).
 template <typename T, int C, int P, int MODE, int TI, int TJ, int TK, int LDS_BYTES>
 __device__ __forceinline__ void sl_tile_body(
     const Grid& g, const ArrivalTabs& A, const TrajVel& V, const double* __restrict__ F,
     const double* __restrict__ Fm, const T* __restrict__ Sn, const T* __restrict__ Sm,
     double dt, double* __restrict__ out, const Feet& ft, unsigned long long* nclamp, long tile,
-    T* __restrict__ lds, int* sh, int* s_cnt) {
+    T* __restrict__ lds, int* sh, int* s_cnt, FootSt* nst = nullptr) {
     constexpr int NPT = TI * TJ * TK;
     static_assert(NPT % kSB == 0, "tile must be a multiple of the block");
     constexpr int PT = NPT / kSB;
@@ -267,6 +295,7 @@ __device__ __forceinline__ void sl_tile_body(
     long idx[PT];
     FootSt st[PT];
     SBox l{INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN};
+    SBox ln{INT_MAX, INT_MIN, INT_MAX, INT_MIN, INT_MAX, INT_MIN};  // MODE_BDF2 near box
     int ncl = 0;
 #pragma unroll
     for (int p = 0; p < PT; ++p) {
@@ -290,15 +319,25 @@ __device__ __forceinline__ void sl_tile_body(
                 // the first sample is at the arrival point for every foot depth
                 double u0, v0, w0;
                 sample3_at(g, VA, A.xdx[I - 1], A.ydy[J - 1], za, A.cell[K - 1], u0, v0, w0);
-                trace_foot(g, VA, xa, ya, za, u0, v0, w0,
-                           MODE == MODE_FAR ? 0.5 * (2.0 * dt) : 0.5 * dt, x, y, z);
-                if (MODE == MODE_FAR) {  // the near foot, for MODE_NEAR
+                if (MODE == MODE_FAR || MODE == MODE_BDF2) {
+                    // the near foot first: it leaves the registers (to LDS or
+                    // to the feet arrays) before the far foot is traced
                     double xn, yn, zn;
                     trace_foot(g, VA, xa, ya, za, u0, v0, w0, 0.5 * dt, xn, yn, zn);
-                    ft.x[n] = xn;
-                    ft.y[n] = yn;
-                    ft.z[n] = zn;
+                    if (MODE == MODE_FAR) {
+                        ft.x[n] = xn;
+                        ft.y[n] = yn;
+                        ft.z[n] = zn;
+                    } else {
+                        FootSt sn;
+                        make_st<C, P>(g, xn, yn, zn, sn, ncl);
+                        box_add(ln, sn);
+                        nst[q] = sn;
+                    }
                 }
+                trace_foot(g, VA, xa, ya, za, u0, v0, w0,
+                           (MODE == MODE_FAR || MODE == MODE_BDF2) ? 0.5 * (2.0 * dt) : 0.5 * dt,
+                           x, y, z);
             }
             make_st<C, P>(g, x, y, z, st[p], ncl);
             box_add(l, st[p]);
@@ -306,18 +345,35 @@ __device__ __forceinline__ void sl_tile_body(
     }
 
     // ---- stage the tile's stencil box in LDS -------------------------------
-    const T* __restrict__ S = (MODE == MODE_FAR) ? Sm : Sn;
+    const T* __restrict__ S = (MODE == MODE_FAR || MODE == MODE_BDF2) ? Sm : Sn;
+    // MODE_BDF2: the near box is reduced now and waits in sh[6..11], so no
+    // lane holds it through the far contraction
+    if (MODE == MODE_BDF2) s_box_reduce(ln, sh + 6);
     const SBox b = s_block_box(l, sh);
     int BY = 0, BZ = 0;
     const bool fits = stage_box<T, C, P>(g, S, b, lds, LDS_BYTES, BY, BZ);
 
     // ---- phase 2: contract -------------------------------------------------
     const double rdx2 = 1.0 / (g.dx * g.dx), rdy2 = 1.0 / (g.dy * g.dy);
+    double farv[PT];
+    if (MODE == MODE_BDF2) {
+        // far value of every point, then the near box into the same buffer
+#pragma unroll
+        for (int p = 0; p < PT; ++p)
+            farv[p] = idx[p] < 0 ? 0.0 : contract<T, C, P>(g, st[p], b, fits, lds, BY, BZ, S);
+        __syncthreads();  // every lane is done with the far box
+    }
+    const SBox bn = (MODE == MODE_BDF2) ? SBox{sh[6], sh[7], sh[8], sh[9], sh[10], sh[11]} : b;
+    const bool fits_n = (MODE == MODE_BDF2) ? stage_box<T, C, P>(g, Sn, bn, lds, LDS_BYTES, BY, BZ)
+                                            : fits;
 #pragma unroll
     for (int p = 0; p < PT; ++p) {
         if (idx[p] < 0) continue;
         const long n = idx[p];
-        const double acc = contract<T, C, P>(g, st[p], b, fits, lds, BY, BZ, S);
+        const double acc =
+            (MODE == MODE_BDF2)
+                ? contract<T, C, P>(g, nst[threadIdx.x + kSB * p], bn, fits_n, lds, BY, BZ, Sn)
+                : contract<T, C, P>(g, st[p], b, fits, lds, BY, BZ, S);
         // arrival ghost index of n
         const unsigned un = (unsigned)n;
         const int K = (int)(un % (unsigned)nzi) + 1;
@@ -331,7 +387,7 @@ __device__ __forceinline__ void sl_tile_body(
         } else if (MODE == MODE_BDF1) {
             out[o] = acc + dt * rhs_xy<C>(g, F, I, J, K, rdx2, rdy2);
         } else {
-            const double far = out[o];
+            const double far = (MODE == MODE_BDF2) ? farv[p] : out[o];
             const double rn = rhs_xy<C>(g, F, I, J, K, rdx2, rdy2);
             const double rm = rhs_xy<C>(g, Fm, I, J, K, rdx2, rdy2);
             const double rh = 2.0 * rn - rm;

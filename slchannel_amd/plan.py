@@ -98,6 +98,7 @@ class HipPlan:
 
     OPT_GATHER_TILE = 1
     OPT_FIELD_STORAGE = 2
+    OPT_BDF2_FUSED = 3
 
     def set_option(self, option, value):
         """Tuning knob (include/slc.h: SLC_OPT_*)."""

@@ -348,16 +348,19 @@ def test_tiled_gather_matches_direct(check, order):
 
 @pytest.mark.parametrize("order", [4, 6])
 def test_tiled_step_kernels_match_direct(check, order):
-    """advect, BDF1 and BDF2 predictors: LDS-tiled kernels (default) against
-    the direct global-load kernels -- identical results and clamp counts."""
+    """advect, BDF1 and BDF2 predictors: LDS-tiled kernels (default, with the
+    fused one-pass BDF2 predictor) against the direct global-load kernels and
+    against the split far/near BDF2 passes -- identical results and clamp
+    counts."""
     c = _random_case(40, 36, 48, order, "symmetric", seed=17, cfl=3.0)
     plan = make_plan(c, nu=1e-3)
     f = [cuda(a) for a in c["fields"]]
     m = [cuda(a) for a in c["mids"]]
     dt = c["dt"]
     res = {}
-    for shape in (-1, 0):
-        plan.set_option(plan.OPT_GATHER_TILE, shape)
+    for shape in (-1, 0, "split"):
+        plan.set_option(plan.OPT_GATHER_TILE, 0 if shape == "split" else shape)
+        plan.set_option(plan.OPT_BDF2_FUSED, 0 if shape == "split" else 1)
         outs = {}
         for name in ("advect", "bdf1", "bdf2"):
             o = [torch.zeros(plan.shapes[k], dtype=torch.float64, device="cuda") for k in "uvw"]
@@ -371,11 +374,16 @@ def test_tiled_step_kernels_match_direct(check, order):
             outs[name] = (o, int(ncl.item()))
         res[shape] = outs
     plan.set_option(plan.OPT_GATHER_TILE, 0)
+    plan.set_option(plan.OPT_BDF2_FUSED, 1)
     for name in ("advect", "bdf1", "bdf2"):
         (a, na), (b, nb) = res[0][name], res[-1][name]
         same = all(torch.equal(x, y) for x, y in zip(a, b))
         dev = max((x - y).abs().max().item() for x, y in zip(a, b))
         check(f"{name}: tiled == direct", same and na == nb, f"max|d|={dev:.1e} ncl={na},{nb}")
+    (a, na), (b, nb) = res[0]["bdf2"], res["split"]["bdf2"]
+    same = all(torch.equal(x, y) for x, y in zip(a, b))
+    dev = max((x - y).abs().max().item() for x, y in zip(a, b))
+    check("bdf2: fused == split", same and na == nb, f"max|d|={dev:.1e} ncl={na},{nb}")
 
 
 @pytest.mark.parametrize("stretching,gamma,nz", [("symmetric", 1.5, 256), ("symmetric", 2.6, 180),
@@ -461,7 +469,17 @@ def test_fp32_storage_error_per_field(check, order):
                 plan.bdf2_predict(*f, *m, dt, *o, ncl)
             outs[name] = (o, int(ncl.item()))
         res[bits] = outs
+    # float storage, split far/near BDF2 passes: the fused pass's doubles
+    plan.set_option(plan.OPT_BDF2_FUSED, 0)
+    o = [torch.zeros(plan.shapes[k], dtype=torch.float64, device="cuda") for k in "uvw"]
+    ncl = torch.zeros(1, dtype=torch.int64, device="cuda")
+    plan.bdf2_predict(*f, *m, dt, *o, ncl)
+    plan.set_option(plan.OPT_BDF2_FUSED, 1)
     plan.set_option(plan.OPT_FIELD_STORAGE, 64)
+    a32, n32 = res[32]["bdf2"]
+    check("fp32 bdf2: fused == split",
+          all(torch.equal(x, y) for x, y in zip(a32, o)) and n32 == int(ncl.item()),
+          f"max|d|={max((x - y).abs().max().item() for x, y in zip(a32, o)):.1e}")
     for name in ("advect", "bdf1", "bdf2"):
         (a, na), (b, nb) = res[32][name], res[64][name]
         scale = base * ((4.0 / 3.0 + 1.0 / 3.0) if name == "bdf2" else 1.0)
