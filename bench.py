@@ -508,6 +508,9 @@ def ensemble_bench(args):
     # (b) batched ensemble
     def build(storage):
         ens = Ensemble(order=args.order, storage=storage)
+        if args.bdf2_split:
+            from slchannel_amd.ensemble import OPT_BDF2_FUSED
+            ens.set_option(OPT_BDF2_FUSED, 0)
         ids = [ens.add(_member_cfg(nx, ny, nz, Re, args.order, storage == 32), dt, fields=ini)
                for (nx, ny, nz, Re, _), dt, ini in zip(specs, dts, inits)]
         return ens, ids

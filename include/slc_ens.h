@@ -83,10 +83,14 @@ int slc_ens_step(slc_ens* ens, void* stream);
  * SLC_ENS_OPT_GRID       workgroups of the non-SL kernels: 0 = one per work unit
  *                        (default), k = 1..32 persistent workgroups per CU
  * SLC_ENS_OPT_GRAPH      1: replay each step from a captured hipGraph (re-captured
- *                        after membership changes), 0: launch eagerly */
+ *                        after membership changes), 0: launch eagerly
+ * SLC_ENS_OPT_BDF2_FUSED 1 (default): one SL tile pass per component (both feet
+ *                        and gathers of the BDF2 members), 0: far pass, then
+ *                        near pass (as SLC_OPT_BDF2_FUSED; same doubles) */
 #define SLC_ENS_OPT_TILE 1
 #define SLC_ENS_OPT_GRID 2
 #define SLC_ENS_OPT_GRAPH 3
+#define SLC_ENS_OPT_BDF2_FUSED 4
 int slc_ens_set_option(slc_ens* ens, int option, int value);
 int slc_ens_get_option(const slc_ens* ens, int option, int* value);
 

@@ -98,6 +98,7 @@ struct slc_ens {
     int tile_shape = 0;
     int grid_per_cu = 0;  // 0: one workgroup per work unit; k: k persistent workgroups per CU
     bool use_graph = false;
+    int bdf2_fused = 1;  // 1: one fused BDF2 tile pass per component; 0: far, then near pass
     int ncu = 256;
     hipStream_t es = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
@@ -453,9 +454,17 @@ int fft_all(slc_ens* e, bool forward) {
     return SLC_OK;
 }
 
-// SL passes of the three components (far, then near / bootstrap)
+// SL passes of the three components: one fused pass each (BDF2 members: both
+// feet and gathers; bootstrap members: BDF1), or far, then near / bootstrap
 int sl_passes(slc_ens* e) {
     const int f32 = e->storage == 32 ? 1 : 0;
+    if (e->bdf2_fused) {
+        for (int c = 0; c < 3; ++c) {
+            EnsLaunch L = launch(e, EK_TILE0 + c, c);
+            HIP_OK(run_ens_tile(L, c, 2, e->order, f32, e->tile_shape, L.total, e->es));
+        }
+        return SLC_OK;
+    }
     for (int c = 0; c < 3; ++c)
         for (int pass = 0; pass < 2; ++pass) {
             EnsLaunch L = launch(e, EK_TILE0 + c, c);
@@ -835,6 +844,13 @@ int slc_ens_set_option(slc_ens* e, int option, int value) {
             e->use_graph = value != 0;
             if (!e->use_graph) drop_graph(e);
             return SLC_OK;
+        case SLC_ENS_OPT_BDF2_FUSED:
+            if (value != 0 && value != 1) return fail(SLC_EINVAL, "bdf2 fused must be 0 or 1");
+            if (value != e->bdf2_fused) {
+                e->bdf2_fused = value;
+                e->dirty = true;  // the captured step holds the other passes
+            }
+            return SLC_OK;
         default:
             return fail(SLC_EINVAL, "unknown ensemble option %d", option);
     }
@@ -847,6 +863,7 @@ int slc_ens_get_option(const slc_ens* e, int option, int* value) {
         case SLC_ENS_OPT_TILE: *value = e->tile_shape; return SLC_OK;
         case SLC_ENS_OPT_GRID: *value = e->grid_per_cu; return SLC_OK;
         case SLC_ENS_OPT_GRAPH: *value = e->use_graph ? 1 : 0; return SLC_OK;
+        case SLC_ENS_OPT_BDF2_FUSED: *value = e->bdf2_fused; return SLC_OK;
         default: return fail(SLC_EINVAL, "unknown ensemble option %d", option);
     }
 }

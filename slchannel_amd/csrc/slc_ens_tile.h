@@ -1,6 +1,7 @@
 // slc_ens_tile.h -- the batched SL tile kernel of the ensemble step:
-// far pass (BDF2 members) and near / bootstrap pass (BDF2 / BDF1 members),
-// each unit one sl_tile_body tile of its member.  Included once per
+// PASS 0 far pass (BDF2 members), PASS 1 near / bootstrap pass (BDF2 / BDF1
+// members), PASS 2 fused pass (MODE_BDF2 / BDF1), each unit one sl_tile_body
+// tile of its member.  Included once per
 // (storage type, order) by slc_ens_tile_{d4,d6,f4,f6}.hip.
 #pragma once
 
@@ -18,13 +19,17 @@ ens_tile_kernel(const EnsMember* __restrict__ mem,
                                                        const unsigned* __restrict__ start,
                                                        const int* __restrict__ umap) {
     __shared__ T lds[kEnsLds / sizeof(T)];
-    __shared__ int sh[6];
+    __shared__ int sh[PASS == 2 ? 12 : 6];
     __shared__ int s_cnt;
+    __shared__ FootSt nst[PASS == 2 ? TI * TJ * TK : 1];
     const unsigned uu = (unsigned)xcd_block(gridDim.x);
-    const int m = umap[uu];
+    // member and state are the same for the whole workgroup: broadcast them
+    // so the member table is read with scalar loads and its pointers stay in
+    // scalar registers
+    const int m = __builtin_amdgcn_readfirstlane(umap[uu]);
     const unsigned u = uu - start[m];
     const EnsMember& M = mem[m];
-    const int st = *M.state;
+    const int st = __builtin_amdgcn_readfirstlane(*M.state);
     const int a = ens_a(st), b = ens_b(st), c = ens_c(st);
     const long NA = (long)M.g.nx * M.g.ny * M.g.nz;
     const Feet ft{M.nfoot, M.nfoot + NA, M.nfoot + 2 * NA};
@@ -44,10 +49,14 @@ ens_tile_kernel(const EnsMember* __restrict__ mem,
         if (PASS == 0)
             sl_tile_body<T, C, P, MODE_FAR, TI, TJ, TK, kEnsLds>(
                 M.g, M.at[C], V, Fa, Fb, Sa, Sb, M.dt, M.fld[c][C], ft, M.ncl, u, lds, sh, &s_cnt);
+        else if (PASS == 2)
+            sl_tile_body<T, C, P, MODE_BDF2, TI, TJ, TK, kEnsLds>(
+                M.g, M.at[C], V, Fa, Fb, Sa, Sb, M.dt, M.fld[c][C], ft, M.ncl, u, lds, sh, &s_cnt,
+                nst);
         else
             sl_tile_body<T, C, P, MODE_NEAR, TI, TJ, TK, kEnsLds>(
                 M.g, M.at[C], V, Fa, Fb, Sa, Sb, M.dt, M.fld[c][C], ft, M.ncl, u, lds, sh, &s_cnt);
-    } else if (PASS == 1) {
+    } else if (PASS >= 1) {
         const TrajVel V{{M.fld[a][0], M.fld[a][1], M.fld[a][2]}, {nullptr, nullptr, nullptr}};
         sl_tile_body<T, C, P, MODE_BDF1, TI, TJ, TK, kEnsLds>(
             M.g, M.at[C], V, Fa, nullptr, Sa, nullptr, M.dt, M.fld[c][C], ft, M.ncl, u, lds, sh,
@@ -75,7 +84,8 @@ inline void launch_tile_shape(const EnsLaunch& L, int shape, unsigned grid, hipS
 template <typename T, int C, int P>
 inline void launch_tile_pass(const EnsLaunch& L, int pass, int shape, unsigned grid, hipStream_t s) {
     if (pass == 0) launch_tile_shape<T, C, P, 0>(L, shape, grid, s);
-    else launch_tile_shape<T, C, P, 1>(L, shape, grid, s);
+    else if (pass == 1) launch_tile_shape<T, C, P, 1>(L, shape, grid, s);
+    else launch_tile_shape<T, C, P, 2>(L, shape, grid, s);
 }
 
 template <typename T, int P>

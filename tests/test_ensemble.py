@@ -126,10 +126,16 @@ def test_join_and_leave_between_steps(check):
 def test_every_tile_shape_and_autotune(check):
     from slchannel_amd.ensemble import OPT_TILE
 
+    from slchannel_amd.ensemble import OPT_BDF2_FUSED
+
     results = {}
-    for shape in (0, 1, 2):
+    for shape in (0, 1, 2, "split"):
         ens = _ensemble()
-        ens.set_option(OPT_TILE, shape)
+        if shape == "split":  # far pass + near pass instead of the fused pass
+            ens.set_option(OPT_BDF2_FUSED, 0)
+            check("split passes selected", ens.get_option(OPT_BDF2_FUSED) == 0, "")
+        else:
+            ens.set_option(OPT_TILE, shape)
         ids = []
         for cfg, seed, dt in MEMBERS[:2]:
             f = _single(cfg, seed)
@@ -148,10 +154,10 @@ def test_every_tile_shape_and_autotune(check):
             st = ens.stats(ids[0]).cpu()
             check("state unchanged by autotune (4 steps)", int(st[3].item()) == 4,
                   f"{st.tolist()}")
-    for shape in (1, 2):
+    for shape in (1, 2, "split"):
         same = all(torch.equal(a[k], b[k]) for a, b in zip(results[shape], results[0])
                    for k in "uvwp")
-        check(f"tile shape {shape} == shape 0", same, "")
+        check(f"tile shape {shape} == shape 0 (fused)", same, "")
 
 
 def test_graph_replay_matches_eager(check):
