@@ -158,6 +158,14 @@ int slc_poisson(slc_plan* plan, const double* div, double rhs_scale, double* p, 
 int slc_project(slc_plan* plan, double* u, double* v, double* w, const double* p, double dt,
                 void* stream);
 
+/* out = a x + b y over the plan's pressure array ((nx+2)(ny+2)(nz+2)
+ * doubles), each product rounded, then the sum: the incremental-pressure
+ * bookkeeping of the 'inc' BDF2 option, p_ext = 2 p^n - p^{n-1} (a = 2,
+ * b = -1; solver.py:913-917) and p^{n+1} = p_ext + phi (a = b = 1;
+ * solver.py:999-1011).  out may alias x or y. */
+int slc_pressure_combine(slc_plan* plan, double* out, double a, const double* x, double b,
+                         const double* y, void* stream);
+
 /* The projection block of one step, fused: div(u)/rhs_scale -> rocFFT ->
  * per-mode Thomas -> inverse rocFFT -> u -= dt_proj grad p -> ghosts of
  * u, v, w and p.  p may be NULL (pressure not materialised).
@@ -223,10 +231,14 @@ int slc_bulk_force(slc_plan* plan, double* u, double* v, double* w, double U_bul
  * LDS-tiled pass per component forms both departure feet and both gathers
  * of slc_bdf2_predict; 0 = a far pass, then a near pass reading the near
  * feet and the far values back from device memory.  Both give the same
- * doubles; applies when SLC_OPT_GATHER_TILE >= 0. */
+ * doubles; applies when SLC_OPT_GATHER_TILE >= 0.
+ * SLC_OPT_BDF2_XY_RHS: the explicit xy term of slc_bdf2_predict, 0 (default)
+ * = 2R^n - R^{n-1} ('extrap'), 1 = R^n ('lagged'); sl.bdf2_xy_rhs,
+ * solver.py:262-266, 848-854. */
 #define SLC_OPT_GATHER_TILE 1
 #define SLC_OPT_FIELD_STORAGE 2
 #define SLC_OPT_BDF2_FUSED 3
+#define SLC_OPT_BDF2_XY_RHS 4
 int slc_set_option(slc_plan* plan, int option, int value);
 
 /* Opt-in launch timing (no counterpart in the reference, whose benchmarks

@@ -687,7 +687,15 @@ class ChannelOracle:
 
     def __init__(self, nx, ny, nz, Lx, Ly, Lz, gamma, Re, U_bulk=1.0, order=6,
                  stretching_type="symmetric", top_wall_bc_type="dirichlet", n_traj_iters=2,
-                 z_f=None, z_c=None, tanh_g=None):
+                 z_f=None, z_c=None, tanh_g=None, bdf2_pressure="noninc",
+                 bdf2_xy_rhs="extrap"):
+        # solver.py:265-281: the two BDF2 options and their checks
+        if bdf2_pressure not in ("noninc", "inc"):
+            raise ValueError(f"sl.bdf2_pressure must be 'noninc' or 'inc', got {bdf2_pressure}")
+        if bdf2_xy_rhs not in ("extrap", "lagged"):
+            raise ValueError(f"sl.bdf2_xy_rhs must be 'extrap' or 'lagged', got {bdf2_xy_rhs}")
+        self.bdf2_pressure, self.bdf2_xy_rhs = bdf2_pressure, bdf2_xy_rhs
+        self._P_curr = self._P_prev = self._p_ext = None
         self.nx, self.ny, self.nz = nx, ny, nz
         self.Lx, self.Ly, self.Lz = Lx, Ly, Lz
         self.nu = 1.0 / Re
@@ -819,19 +827,22 @@ class ChannelOracle:
         self.apply_bc_uvw()
         self._zsolve_all(dt)
         self._project(dt, dt)
+        if self.bdf2_pressure == "inc":  # solver.py:802-807: p^1 = phi, no p^0
+            self._P_curr = self.p.copy()
+            self._P_prev = None
         self._dt_prev_step = dt
         return self._apply_bulk_forcing(dt)
 
-    def step_sl_bdf2(self, dt):
-        """solver.py:813-1018 (bdf2_pressure='noninc', bdf2_xy_rhs='extrap')."""
+    def bdf2_predictor(self, dt):
+        """solver.py:846-903: the BDF2 predictor (4/3 near - 1/3 far + (2dt/3)
+        Rh) of the current state and history, Rh = 2R^n - R^{n-1} ('extrap')
+        or R^n ('lagged').  Leaves the state alone; returns (us, vs, ws, R^n)."""
         nx, ny, nz = self.nx, self.ny, self.nz
-        self.apply_bc_uvw()
-        if (self.u_nm1 is None or self._dt_prev_step is None
-                or abs(dt - self._dt_prev_step) > 1e-12 * dt):
-            return self._step_sl_bdf1_bootstrap(dt)
         rhs = self._explicit_xy_rhs()
-        Rh = tuple(2.0 * r - rn for r, rn in zip(rhs, self._Rxy_nm1))
-        self._Rxy_nm1 = tuple(r.copy() for r in rhs)
+        if self.bdf2_xy_rhs == "extrap":
+            Rh = tuple(2.0 * r - rn for r, rn in zip(rhs, self._Rxy_nm1))
+        else:
+            Rh = rhs
         mid = tuple(2.0 * a - b for a, b in zip((self.u, self.v, self.w), self.u_nm1))
         far = self.sl.advect(*self.u_nm1, *mid, 2.0 * dt)
         far = tuple(a.copy() for a in far)
@@ -843,10 +854,51 @@ class ChannelOracle:
         us[1 : nx + 1, 1 : ny + 1, 1 : nz + 1] += dt_eff * Rh[0][1 : nx + 1, 1 : ny + 1, 1 : nz + 1]
         vs[1 : nx + 1, 1 : ny + 1, 1 : nz + 1] += dt_eff * Rh[1][1 : nx + 1, 1 : ny + 1, 1 : nz + 1]
         ws[1 : nx + 1, 1 : ny + 1, 1:nz] += dt_eff * Rh[2][1 : nx + 1, 1 : ny + 1, 1:nz]
+        return us, vs, ws, rhs
+
+    def pressure_extrapolation(self):
+        """solver.py:906-917: p_ext = 2 p^n - p^{n-1}, or p^n after the
+        bootstrap ('inc' only; None before any step)."""
+        if self.bdf2_pressure != "inc" or self._P_curr is None:
+            return None
+        if self._P_prev is None:
+            return self._P_curr
+        return self._P_curr * 2.0 + (-1.0) * self._P_prev
+
+    def step_sl_bdf2(self, dt):
+        """solver.py:813-1018, both BDF2 options (bdf2_pressure 'noninc' /
+        'inc', bdf2_xy_rhs 'extrap' / 'lagged')."""
+        nx, ny, nz = self.nx, self.ny, self.nz
+        self.apply_bc_uvw()
+        if (self.u_nm1 is None or self._dt_prev_step is None
+                or abs(dt - self._dt_prev_step) > 1e-12 * dt):
+            return self._step_sl_bdf1_bootstrap(dt)
+        us, vs, ws, rhs = self.bdf2_predictor(dt)
+        self._Rxy_nm1 = tuple(r.copy() for r in rhs)
+        dt_eff = (2.0 / 3.0) * dt
+        p_ext = self.pressure_extrapolation()
+        if p_ext is not None:
+            # solver.py:918-941: - grad p_ext (project_velocity of zeros with
+            # dt = 1), added with dt_eff before the implicit solve
+            gp = [np.zeros_like(a) for a in (us, vs, ws)]
+            project_velocity(*gp, p_ext, nx, ny, nz, self.dx, self.dy, self.dz_c, self.dz_f,
+                             1.0)
+            us += dt_eff * gp[0]
+            vs += dt_eff * gp[1]
+            ws += dt_eff * gp[2]
         self.u_nm1 = (self.u.copy(), self.v.copy(), self.w.copy())
         self.u, self.v, self.w = us, vs, ws
         self.apply_bc_uvw()
         self._zsolve_all(dt_eff)
         self._project(dt_eff, dt_eff)
+        if self.bdf2_pressure == "inc":
+            # solver.py:999-1012: p^{n+1} = p_ext + phi, history rotated
+            phi = self.p
+            if self._P_curr is None:
+                self._P_curr = phi.copy()
+            else:
+                self._P_prev = self._P_curr
+                self._P_curr = p_ext + phi
+            self.p = self._P_curr
         self._dt_prev_step = dt
         return self._apply_bulk_forcing(dt)

@@ -19,6 +19,7 @@ EXPORTS = (
     "slc_plan_workspace_bytes", "slc_apply_bc", "slc_departure", "slc_gather",
     "slc_stencil", "slc_locate_z", "slc_advect", "slc_bdf1_predict", "slc_bdf2_predict",
     "slc_zsolve", "slc_divergence", "slc_poisson", "slc_project", "slc_project_step",
+    "slc_pressure_combine",
     "slc_bulk_velocity", "slc_bulk_force", "slc_diagnostics", "slc_imex_rhs",
     "slc_ab2_update", "slc_set_option",
     "slc_profile_enable", "slc_profile_report",
@@ -79,6 +80,7 @@ def load():
         "slc_divergence": ([vp, vp, vp, vp, vp, vp], i),
         "slc_poisson": ([vp, vp, d, vp, vp], i),
         "slc_project": ([vp, vp, vp, vp, vp, d, vp], i),
+        "slc_pressure_combine": ([vp, vp, d, vp, d, vp, vp], i),
         "slc_project_step": ([vp, vp, vp, vp, d, d, vp, vp], i),
         "slc_bulk_velocity": ([vp, vp, vp, vp], i),
         "slc_bulk_force": ([vp, vp, vp, vp, d, d, vp, vp], i),

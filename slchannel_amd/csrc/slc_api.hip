@@ -189,6 +189,7 @@ int slc::plan_create(const slc_grid_desc* d, int device, bool spectral, slc_plan
     g.n_iters = d->n_traj_iters;
     g.symmetric = d->stretching == 0 ? 1 : 0;
     g.top_neumann = d->top_wall == 1 ? 1 : 0;
+    g.xy_lagged = 0;
     g.dx = dx;
     g.dy = dy;
     g.Lz = d->Lz;
@@ -586,6 +587,16 @@ int slc_project(slc_plan* p, double* u, double* v, double* w, const double* pr, 
     return SLC_OK;
 }
 
+int slc_pressure_combine(slc_plan* p, double* out, double a, const double* x, double b,
+                         const double* y, void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (!out || !x || !y) return fail(SLC_EINVAL, "null array");
+    DeviceGuard dg(p->device);
+    const long n = (long)(p->nx + 2) * (p->ny + 2) * (p->nz + 2);
+    HIP_OK(run_axpby(out, x, y, a, b, n, S(stream)));
+    return SLC_OK;
+}
+
 int slc_project_step(slc_plan* p, double* u, double* v, double* w, double rhs_scale,
                      double dt_proj, double* pr, void* stream) {
     if (int rc = check_plan(p)) return rc;
@@ -635,6 +646,10 @@ int slc_set_option(slc_plan* p, int option, int value) {
         case SLC_OPT_GATHER_TILE:
             if (value < -1 || value > 3) return fail(SLC_EINVAL, "gather tile shape %d", value);
             p->gather_shape = value;
+            return SLC_OK;
+        case SLC_OPT_BDF2_XY_RHS:
+            if (value != 0 && value != 1) return fail(SLC_EINVAL, "bdf2 xy rhs must be 0 or 1");
+            p->g.xy_lagged = value;
             return SLC_OK;
         case SLC_OPT_BDF2_FUSED:
             if (value != 0 && value != 1) return fail(SLC_EINVAL, "bdf2 fused must be 0 or 1");

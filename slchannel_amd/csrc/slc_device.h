@@ -30,6 +30,7 @@ struct Grid {
     int n_iters;      // iterated-midpoint fixed-point iterations
     int symmetric;    // 1: symmetric tanh map, 0: 'bottom'
     int top_neumann;  // top wall: 0 no-slip (odd ghost), 1 free-slip (even ghost)
+    int xy_lagged;    // BDF2 predictor xy RHS: 0 2R^n - R^{n-1} ('extrap'), 1 R^n ('lagged')
     double dx, dy, Lz, gamma, tanh_g, z_lo, z_hi, nu;
     const double* z_c;      // nz+2 centres incl. ghost centres
     const double* z_f;      // nz+1 faces

@@ -249,8 +249,8 @@ __global__ __launch_bounds__(kBlock) void bdf2_near_kernel(Grid g, TrajVel V,
         const long o = out_addr<C>(g, I, J, K);
         double far = out[o];
         double rn = rhs_xy<C>(g, Fn, I, J, K, dx2, dy2);
-        double rm = rhs_xy<C>(g, Fm, I, J, K, dx2, dy2);
-        double rh = 2.0 * rn - rm;
+        // Rh: 2R^n - R^{n-1} ('extrap') or R^n ('lagged'), solver.py:848-854
+        double rh = g.xy_lagged ? rn : 2.0 * rn - rhs_xy<C>(g, Fm, I, J, K, dx2, dy2);
         double pred = fma(far, -1.0 / 3.0, near * (4.0 / 3.0));
         out[o] = pred + dt_eff * rh;
     }

@@ -389,8 +389,8 @@ __device__ __forceinline__ void sl_tile_body(
         } else {
             const double far = (MODE == MODE_BDF2) ? farv[p] : out[o];
             const double rn = rhs_xy<C>(g, F, I, J, K, rdx2, rdy2);
-            const double rm = rhs_xy<C>(g, Fm, I, J, K, rdx2, rdy2);
-            const double rh = 2.0 * rn - rm;
+            // Rh: 2R^n - R^{n-1} ('extrap') or R^n ('lagged'), solver.py:848-854
+            const double rh = g.xy_lagged ? rn : 2.0 * rn - rhs_xy<C>(g, Fm, I, J, K, rdx2, rdy2);
             const double dt_eff = (2.0 / 3.0) * dt;
             const double pred = fma(far, -1.0 / 3.0, acc * (4.0 / 3.0));
             out[o] = pred + dt_eff * rh;

@@ -29,6 +29,10 @@ def _config_grid(cfg):
     g, dom, flow = cfg["grid"], cfg["domain"], cfg["flow"]
     sl = cfg.get("sl", {})
     bc = cfg.get("boundary_conditions", {})
+    if sl.get("bdf2_pressure", "noninc") != "noninc" or sl.get("bdf2_xy_rhs", "extrap") != "extrap":
+        raise ValueError("ensemble members run the production BDF2 options (bdf2_pressure "
+                         "'noninc', bdf2_xy_rhs 'extrap'); 'inc' / 'lagged' are single-run "
+                         "options of SLChannelFlow")
     return dict(
         nx=int(g["nx"]), ny=int(g["ny"]), nz=int(g["nz"]),
         Lx=float(dom["Lx"]), Ly=float(dom["Ly"]), Lz=float(dom["Lz"]),

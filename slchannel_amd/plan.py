@@ -99,6 +99,7 @@ class HipPlan:
     OPT_GATHER_TILE = 1
     OPT_FIELD_STORAGE = 2
     OPT_BDF2_FUSED = 3
+    OPT_BDF2_XY_RHS = 4
 
     def set_option(self, option, value):
         """Tuning knob (include/slc.h: SLC_OPT_*)."""
@@ -252,6 +253,14 @@ class HipPlan:
         self._chk(p, self.shapes["p"], "p")
         _lib.check(self._lib.slc_project(self._h, _ptr(u), _ptr(v), _ptr(w), _ptr(p), float(dt),
                                          self._stream()), "slc_project")
+
+    def pressure_combine(self, out, a, x, b, y):
+        """out = a x + b y over pressure-shaped arrays (include/slc.h)."""
+        for t, name in ((out, "out"), (x, "x"), (y, "y")):
+            self._chk(t, self.shapes["p"], name)
+        _lib.check(self._lib.slc_pressure_combine(self._h, _ptr(out), float(a), _ptr(x), float(b),
+                                                  _ptr(y), self._stream()),
+                   "slc_pressure_combine")
 
     def project_step(self, u, v, w, rhs_scale, dt_proj, p=None):
         self._uvw(u, v, w)

@@ -70,11 +70,16 @@ class SLChannelFlow:
         self.sl_traj_order = int(sl.get("traj_interp_order", 2))
         self.sl_traj_iters = int(sl.get("n_traj_iters", 2))
         self.sl_interp_dtype = sl.get("interp_dtype", "fp64")
+        # BDF2 options (solver.py:256-281): 'noninc' p^{n+1} = phi, or 'inc'
+        # on p_ext = 2 p^n - p^{n-1}; xy RHS 2R^n - R^{n-1} ('extrap') or R^n
         self.sl_bdf2_pressure = sl.get("bdf2_pressure", "noninc")
         self.sl_bdf2_xy_rhs = sl.get("bdf2_xy_rhs", "extrap")
-        if self.sl_bdf2_pressure != "noninc" or self.sl_bdf2_xy_rhs != "extrap":
-            raise ValueError("the HIP layer implements bdf2_pressure='noninc' with "
-                             "bdf2_xy_rhs='extrap' (the production defaults)")
+        if self.sl_bdf2_pressure not in ("noninc", "inc"):
+            raise ValueError(
+                f"sl.bdf2_pressure must be 'noninc' or 'inc', got {self.sl_bdf2_pressure}")
+        if self.sl_bdf2_xy_rhs not in ("extrap", "lagged"):
+            raise ValueError(
+                f"sl.bdf2_xy_rhs must be 'extrap' or 'lagged', got {self.sl_bdf2_xy_rhs}")
 
         if device is None:
             device = torch.device("cuda", torch.cuda.current_device()) \
@@ -106,6 +111,8 @@ class SLChannelFlow:
                             n_traj_iters=self.sl_traj_iters,
                             top_wall_bc_type=self.top_wall_bc_type, nu=self.nu,
                             device=self.device)
+        if self.sl_bdf2_xy_rhs == "lagged":
+            self.plan.set_option(HipPlan.OPT_BDF2_XY_RHS, 1)
         self.sl = SLAdvector(self.nx, self.ny, self.nz, self.dx, self.dy, self.Lx, self.Ly,
                              self.Lz, self.z_f, self.z_c, self.gamma,
                              stretching_type=self.stretching_type, order=self.sl_order,
@@ -118,6 +125,8 @@ class SLChannelFlow:
         self._star = None
         self._dt_prev_step = None
         self._imex_cur = self._imex_prev = None  # IMEX rhs buffers (step_imex)
+        # 'inc' pressure: Poisson output phi, full levels p^n / p^{n-1}, p_ext
+        self._phi = self._P_curr = self._P_prev = self._p_ext = None
         self.forcing = 0.0
         self.current_step = 0
         self.time = 0.0
@@ -212,14 +221,26 @@ class SLChannelFlow:
         self.forcing = self._bulk[1].clone()
         return self._bulk[2].clone(), self.forcing
 
-    def _implicit_and_project(self, dt_z, rhs_scale, dt_proj, theta=1.0):
+    def _implicit_and_project(self, dt_z, rhs_scale, dt_proj, theta=1.0, sl=True):
+        """Ghosts, z-solves, ghosts, projection; returns the buffer holding
+        phi: self.p, or for SL steps with the 'inc' option a separate phi
+        buffer (self.p then ends as the full pressure p^{n+1})."""
         plan = self.plan
         plan.apply_bc(self.u, self.v, self.w)
         plan.zsolve("u", self.u, dt_z, theta)
         plan.zsolve("v", self.v, dt_z, theta)
         plan.zsolve("w", self.w, dt_z, theta)
         plan.apply_bc(self.u, self.v, self.w)
-        plan.project_step(self.u, self.v, self.w, rhs_scale, dt_proj, self.p)
+        if sl and self.sl_bdf2_pressure == "inc":
+            if self._phi is None:
+                self._phi = torch.zeros_like(self.p)
+            phi = self._phi
+        else:
+            if self.p is self._P_curr or self.p is self._P_prev:
+                self.p = torch.empty_like(self.p)  # keep the 'inc' history intact
+            phi = self.p
+        plan.project_step(self.u, self.v, self.w, rhs_scale, dt_proj, phi)
+        return phi
 
     def _step_sl_bdf1_bootstrap(self, dt):
         """solver.py:729-811: U* = V^n, one foot at dt, theta = 1 at dt."""
@@ -236,7 +257,15 @@ class SLChannelFlow:
         # adopt the predictor buffers; the old field buffers become scratch
         self._star = [self.u, self.v, self.w]
         self.u, self.v, self.w = us, vs, ws
-        self._implicit_and_project(dt, dt, dt)
+        phi = self._implicit_and_project(dt, dt, dt)
+        if self.sl_bdf2_pressure == "inc":
+            # solver.py:802-808: p^n = phi, no p^{n-1} yet
+            if self._P_curr is None:
+                self._P_curr = phi.clone()
+            else:
+                self._P_curr.copy_(phi)
+            self._P_prev = None
+            self.p = self._P_curr
         self.sl.n_clamped_last = self._nclamp[0].clone()
         self._dt_prev_step = dt
         return self._apply_bulk_forcing(dt)
@@ -257,18 +286,48 @@ class SLChannelFlow:
             us, vs, ws = self._star
             self.plan.bdf2_predict(self.u, self.v, self.w, self.u_nm1, self.v_nm1, self.w_nm1,
                                    dt, us, vs, ws, self._nclamp)
+            dt_eff = (2.0 / 3.0) * dt
+            p_ext = self._pressure_extrapolation()
+            if p_ext is not None:
+                # solver.py:918-941: predictor += dt_eff * (-grad p_ext) before
+                # the implicit solve; u -= dt_eff grad p_ext is the same doubles
+                self.plan.project(us, vs, ws, p_ext, dt_eff)
             # rotate: V^{n-1} <- V^n, V <- predictor, scratch <- old V^{n-1}
             self._star = [self.u_nm1, self.v_nm1, self.w_nm1]
             self.u_nm1, self.v_nm1, self.w_nm1 = self.u, self.v, self.w
             self.u, self.v, self.w = us, vs, ws
-            dt_eff = (2.0 / 3.0) * dt
-            self._implicit_and_project(dt_eff, dt_eff, dt_eff)
+            phi = self._implicit_and_project(dt_eff, dt_eff, dt_eff)
+            if self.sl_bdf2_pressure == "inc":
+                self._pressure_update(p_ext, phi)
             self.sl.n_clamped_last = self._nclamp[0].clone()
             self._dt_prev_step = dt
             out = self._apply_bulk_forcing(dt)
         self.current_step += 1
         self.time += dt
         return out
+
+    def _pressure_extrapolation(self):
+        """solver.py:906-917 ('inc'): p_ext = 2 p^n - p^{n-1}, or p^n when
+        there is no p^{n-1} (first BDF2 step after a bootstrap)."""
+        if self.sl_bdf2_pressure != "inc" or self._P_curr is None:
+            return None
+        if self._P_prev is None:
+            return self._P_curr
+        if self._p_ext is None:
+            self._p_ext = torch.empty_like(self._P_curr)
+        self.plan.pressure_combine(self._p_ext, 2.0, self._P_curr, -1.0, self._P_prev)
+        return self._p_ext
+
+    def _pressure_update(self, p_ext, phi):
+        """solver.py:999-1012 ('inc'): p^{n+1} = p_ext + phi, p^{n-1} <- p^n."""
+        if self._P_curr is None:
+            self._P_curr = phi.clone()
+        else:
+            # the old p^{n-1} buffer is free once p_ext is formed
+            nxt = self._P_prev if self._P_prev is not None else torch.empty_like(phi)
+            self.plan.pressure_combine(nxt, 1.0, p_ext, 1.0, phi)
+            self._P_prev, self._P_curr = self._P_curr, nxt
+        self.p = self._P_curr
 
     # ------------------------------------------------------------------
     # Eulerian IMEX reference scheme (solver.py:1024-1127)
@@ -298,7 +357,7 @@ class SLChannelFlow:
         self.plan.ab2_update(self.u, self.v, self.w, self._imex_cur, prev, dt)
         # the current rhs becomes the previous one (buffers swapped, no copy)
         self._imex_prev, self._imex_cur = self._imex_cur, prev
-        self._implicit_and_project(dt, dt, dt, theta=0.5)
+        self._implicit_and_project(dt, dt, dt, theta=0.5, sl=False)
         out = self._apply_bulk_forcing(dt)
         self.current_step += 1
         self.time += dt

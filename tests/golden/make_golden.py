@@ -3,7 +3,9 @@
 Run ONCE in the build container (the reference checkout is not present on the
 GPU machines):
 
-    MPLBACKEND=Agg python tests/golden/make_golden.py /root/reference
+    MPLBACKEND=Agg python tests/golden/make_golden.py /root/reference [name ...]
+
+(names, e.g. ``step_o4_inc``, regenerate only those files.)
 
 It imports the reference package (gmcavallazzi/SLchannel, pure Python/torch)
 from the given checkout and evaluates, on small seeded inputs, exactly the
@@ -18,7 +20,9 @@ functions whose results no closed form or invariant fixes bit-for-bit:
 * ``solve_*.npz``    solve_poisson_fft (projection.py:133-197) and
                      solve_implicit_diffusion_{u,v,w} (operators.py:382-630);
 * ``step_*.npz``     SLChannelFlow.step_sl_bdf2 (solver.py:813-1018), BDF1
-                     bootstrap included, from a seeded projected state.
+                     bootstrap included, from a seeded projected state; two
+                     cases with the non-default BDF2 options (bdf2_pressure
+                     'inc', bdf2_xy_rhs 'lagged', solver.py:256-281).
 
 Inputs are generated here with numpy (seeded) and stored next to the outputs,
 so the fixtures are self-contained: data only, no reference code.
@@ -206,7 +210,8 @@ def gen_solve(ref):
     np.savez_compressed(os.path.join(HERE, "solve_small.npz"), **d)
 
 
-def gen_step(ref, name, order, stretching="symmetric", top="dirichlet", n_steps=4, seed=3):
+def gen_step(ref, name, order, stretching="symmetric", top="dirichlet", n_steps=4, seed=3,
+             sl_extra=None):
     torch, semilag, solver_mod, utils, projection, operators = ref
     nx, ny, nz = 12, 10, 16
     Lx, Ly, Lz, gamma = 4 * math.pi, 2 * math.pi, 2.0, 1.5
@@ -227,7 +232,8 @@ def gen_step(ref, name, order, stretching="symmetric", top="dirichlet", n_steps=
             "output": {"results_folder": os.path.join(tmp, "results"), "n_out": 10**8,
                        "n_save": 10**8},
             "advection": {"scheme": "sl"},
-            "sl": {"interp_order": order, "n_traj_iters": 2, "interp_dtype": "fp64"},
+            "sl": {"interp_order": order, "n_traj_iters": 2, "interp_dtype": "fp64",
+                   **(sl_extra or {})},
             "statistics": {"n_stats": 0},
         }
         path = os.path.join(tmp, "c.yaml")
@@ -254,6 +260,8 @@ def gen_step(ref, name, order, stretching="symmetric", top="dirichlet", n_steps=
         d = {
             "nx": nx, "ny": ny, "nz": nz, "Lx": Lx, "Ly": Ly, "Lz": Lz, "gamma": gamma,
             "Re": Re, "dt": dt, "order": order, "stretching": stretching, "top": top,
+            "bdf2_pressure": (sl_extra or {}).get("bdf2_pressure", "noninc"),
+            "bdf2_xy_rhs": (sl_extra or {}).get("bdf2_xy_rhs", "extrap"),
             "z_f": zf, "z_c": zc,
             "u0": tu.numpy().copy(), "v0": tv.numpy().copy(), "w0": tw.numpy().copy(),
             "p0": p.numpy().copy(),
@@ -278,17 +286,28 @@ def gen_step(ref, name, order, stretching="symmetric", top="dirichlet", n_steps=
 
 def main():
     ref_path = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    only = set(sys.argv[2:])
     ref = _import_reference(ref_path)
-    gen_grid(ref, "sym_g15_n16", 1.5, 16, 2.0, "symmetric")
-    gen_grid(ref, "bot_g18_n24", 1.8, 24, 2.0, "bottom")
-    gen_grid(ref, "sym_g15_n64", 1.5, 64, 2.0, "symmetric")
-    gen_advect(ref, "o4", 4)
-    gen_advect(ref, "o6", 6)
-    gen_advect(ref, "o6_bottom", 6, stretching="bottom", seed=2)
-    gen_advect(ref, "o4_clamp", 4, clamp=True, seed=5)
-    gen_solve(ref)
-    gen_step(ref, "o6", 6)
-    gen_step(ref, "o4_bottom_neumann", 4, stretching="bottom", top="neumann", seed=4)
+    cases = [
+        ("grid_sym_g15_n16", lambda: gen_grid(ref, "sym_g15_n16", 1.5, 16, 2.0, "symmetric")),
+        ("grid_bot_g18_n24", lambda: gen_grid(ref, "bot_g18_n24", 1.8, 24, 2.0, "bottom")),
+        ("grid_sym_g15_n64", lambda: gen_grid(ref, "sym_g15_n64", 1.5, 64, 2.0, "symmetric")),
+        ("advect_o4", lambda: gen_advect(ref, "o4", 4)),
+        ("advect_o6", lambda: gen_advect(ref, "o6", 6)),
+        ("advect_o6_bottom", lambda: gen_advect(ref, "o6_bottom", 6, stretching="bottom", seed=2)),
+        ("advect_o4_clamp", lambda: gen_advect(ref, "o4_clamp", 4, clamp=True, seed=5)),
+        ("solve_small", lambda: gen_solve(ref)),
+        ("step_o6", lambda: gen_step(ref, "o6", 6)),
+        ("step_o4_bottom_neumann", lambda: gen_step(ref, "o4_bottom_neumann", 4,
+                                                    stretching="bottom", top="neumann", seed=4)),
+        ("step_o4_inc", lambda: gen_step(ref, "o4_inc", 4, seed=6,
+                                         sl_extra={"bdf2_pressure": "inc"})),
+        ("step_o6_lagged", lambda: gen_step(ref, "o6_lagged", 6, seed=7,
+                                            sl_extra={"bdf2_xy_rhs": "lagged"})),
+    ]
+    for name, make in cases:
+        if not only or name in only:
+            make()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -253,12 +253,14 @@ def _flow_from_golden(g):
         "flow": {"Re": float(g["Re"]), "U_bulk": 1.0, "gamma": float(g["gamma"])},
         "boundary_conditions": {"top_wall": {"type": str(g["top"])}},
         "time": {"dt": float(g["dt"])},
-        "sl": {"interp_order": int(g["order"]), "n_traj_iters": 2},
+        "sl": {"interp_order": int(g["order"]), "n_traj_iters": 2,
+               "bdf2_pressure": str(g["bdf2_pressure"]) if "bdf2_pressure" in g else "noninc",
+               "bdf2_xy_rhs": str(g["bdf2_xy_rhs"]) if "bdf2_xy_rhs" in g else "extrap"},
     }
     return SLChannelFlow(config=cfg, z_tables=(g["z_f"], g["z_c"]))
 
 
-@pytest.mark.parametrize("name", ["o6", "o4_bottom_neumann"])
+@pytest.mark.parametrize("name", ["o6", "o4_bottom_neumann", "o4_inc", "o6_lagged"])
 def test_step_vs_reference(check, golden, name):
     g = golden("step_" + name)
     flow = _flow_from_golden(g)

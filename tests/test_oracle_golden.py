@@ -123,14 +123,21 @@ def test_solves_vs_reference(check, golden, tag):
             check(f"zsolve {nm} theta={theta}", e < 1e-13, f"rel={e:.1e}")
 
 
-@pytest.mark.parametrize("name", ["o6", "o4_bottom_neumann"])
+def _bdf2_options(g):
+    """The fixture's BDF2 options (the defaults for files without them)."""
+    return {k: str(g[k]) if k in g else d
+            for k, d in (("bdf2_pressure", "noninc"), ("bdf2_xy_rhs", "extrap"))}
+
+
+@pytest.mark.parametrize("name", ["o6", "o4_bottom_neumann", "o4_inc", "o6_lagged"])
 def test_step_vs_reference(check, golden, name):
     g = golden("step_" + name)
     nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
     co = orc.ChannelOracle(nx, ny, nz, float(g["Lx"]), float(g["Ly"]), float(g["Lz"]),
                            float(g["gamma"]), float(g["Re"]), order=int(g["order"]),
                            stretching_type=str(g["stretching"]),
-                           top_wall_bc_type=str(g["top"]), z_f=g["z_f"], z_c=g["z_c"])
+                           top_wall_bc_type=str(g["top"]), z_f=g["z_f"], z_c=g["z_c"],
+                           **_bdf2_options(g))
     co.set_fields(g["u0"], g["v0"], g["w0"], g["p0"])
     dt = float(g["dt"])
     n_steps = int(g["n_steps"])
