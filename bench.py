@@ -198,9 +198,11 @@ def main():
                                f"Re_b {Re_b} dt+ {dtp}",
                    "nx": n, "ny": n, "nz": nz, "interp_order": args.order, "dt": dt,
                    "storage_bits": args.storage},
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
+                     "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": traffic,
-                     "bytes_per_launch": bytes_far, "avg_ms": round(avg_ms, 4)},
+                     "bytes_per_launch": bytes_far, "avg_ms": round(avg_ms, 4),
+                     "fp64_check": fp64_check(lab, avg_ms, N_u, args.order)},
         "gather_GBps": round(gather_gbps, 1),
         "gather_frac_hbm": round(gather_gbps / PEAK_HBM_GBPS, 4),
         "max_div": div,
@@ -401,7 +403,7 @@ def large_box(args):
     prof = plan.profile_report()
     plan.profile(False)
     N_u = nx * ny * nz
-    _, kname, avg_ms, bytes_k = roofline_kernel(prof, N_u, 64)
+    lab, kname, avg_ms, bytes_k = roofline_kernel(prof, N_u, 64)
     achieved = bytes_k / (avg_ms * 1e-3) / 1e9
     # isolated gather, feet = arrival + U(-5, 5) cells in x and y
     g = torch.Generator(device="cuda").manual_seed(33)
@@ -437,7 +439,8 @@ def large_box(args):
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": None,
-                     "bytes_per_launch": bytes_k, "avg_ms": round(avg_ms, 4)},
+                     "bytes_per_launch": bytes_k, "avg_ms": round(avg_ms, 4),
+                     "fp64_check": fp64_check(lab, avg_ms, N_u, args.order)},
         "gather_random_pm5_GBps": round(gather_gbps, 1),
         "max_div": div, "u_bulk": ub.item(),
         "kernels_ms": {k: {"launches": c, "avg_ms": round(t / c, 4)}
@@ -646,6 +649,35 @@ def roofline_kernel(prof, n_pts, storage):
         per_pt = 64.0 if storage == 64 else 60.0
     kname = "sl_tile_kernel<u,P,%s> (%s)" % ("BDF2" if fused else "FAR", lab)
     return lab, kname, tot / cnt, per_pt * n_pts
+
+
+PEAK_FP64_TFLOPS = 78.6
+
+
+def fp64_flops_per_point(order, fused):
+    """FP64 flops per arrival point of the dominant SL kernel, counted from
+    the algorithm (FMA = 2).  Gather at order P: the staged contraction
+    2 (P^3 + P^2 + P), Lagrange weights P^2 per uniform dimension (P
+    differences, P products of P - 1 factors and a scale) and P^2 for z.
+    Tracer: 9 trilinear samples of U* (arrival point + one midpoint per foot
+    at n_traj_iters = 2; 41 flops each: coordinates, 2-point weights, eight
+    weighted terms) and 52 flops of midpoints, feet and stencil origins.
+    Fused BDF2 pass: two gathers, the tracer, the two xy-RHS stencils and the
+    BDF2 combination (27).  Far pass: one gather and the tracer.  (At order 6
+    the fused count is 1696; the SQ_INSTS_VALU_FLOPS_FP64 counter of the same
+    launch gives 1750.)"""
+    P = order
+    gather = 2 * (P**3 + P**2 + P) + 3 * P**2
+    tracer = 9 * 41 + 52
+    return 2 * gather + tracer + 27 if fused else gather + tracer
+
+
+def fp64_check(prof_label, avg_ms, n_pts, order):
+    """The HBM roofline of the SL pass checked against the FP64 VALU peak."""
+    flops = fp64_flops_per_point(order, prof_label == "bdf2_u") * n_pts
+    tf = flops / (avg_ms * 1e-3) / 1e12
+    return {"flops_per_launch": flops, "achieved": round(tf, 2), "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(tf / PEAK_FP64_TFLOPS, 4)}
 
 
 def measured_traffic(label, n, nz, order):
