@@ -69,6 +69,8 @@ class _OrcGrid(ctypes.Structure):
         ("z_c", ctypes.POINTER(ctypes.c_double)), ("z_f", ctypes.POINTER(ctypes.c_double)),
         ("dinv_c2", ctypes.POINTER(ctypes.c_double)), ("dinv_f2", ctypes.POINTER(ctypes.c_double)),
         ("dinv_cP", ctypes.POINTER(ctypes.c_double)), ("dinv_fP", ctypes.POINTER(ctypes.c_double)),
+        ("traj_order", ctypes.c_int),
+        ("dinv_cT", ctypes.POINTER(ctypes.c_double)), ("dinv_fT", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -134,9 +136,13 @@ class SLOracle:
     """Restates SLAdvector (semilag.py:130-671) on the fp64 eager path."""
 
     def __init__(self, nx, ny, nz, dx, dy, Lx, Ly, Lz, z_f, z_c, gamma,
-                 stretching_type="symmetric", order=4, n_traj_iters=2):
+                 stretching_type="symmetric", order=4, n_traj_iters=2, traj_order=2):
         if order not in (4, 6):
             raise ValueError(f"order must be 4 or 6, got {order}")
+        if traj_order not in (2, 4):  # semilag.py:206-209
+            raise ValueError(
+                f"sl.traj_interp_order must be 2 (trilinear) or 4 (tricubic), got {traj_order}")
+        self.traj_order = traj_order
         self.nx, self.ny, self.nz = nx, ny, nz
         self.dx, self.dy, self.Lx, self.Ly, self.Lz = dx, dy, Lx, Ly, Lz
         self.gamma = float(gamma)
@@ -153,6 +159,8 @@ class SLOracle:
             ("f", 2): z_denominator_table(self.z_f, 2),
             ("c", order): z_denominator_table(self.z_c, order),
             ("f", order): z_denominator_table(self.z_f, order),
+            ("c", traj_order): z_denominator_table(self.z_c, traj_order),
+            ("f", traj_order): z_denominator_table(self.z_f, traj_order),
         }
         self._g = _OrcGrid(
             nx, ny, nz, order, n_traj_iters, 1 if stretching_type == "symmetric" else 0,
@@ -160,6 +168,7 @@ class SLOracle:
             _dptr(self.z_c), _dptr(self.z_f),
             _dptr(self.dinv[("c", 2)]), _dptr(self.dinv[("f", 2)]),
             _dptr(self.dinv[("c", order)]), _dptr(self.dinv[("f", order)]),
+            traj_order, _dptr(self.dinv[("c", traj_order)]), _dptr(self.dinv[("f", traj_order)]),
         )
         self.n_clamped_last = 0
 
@@ -214,7 +223,8 @@ class SLOracle:
         return out, int(ncl)
 
     def sample(self, comp, field, x, y, z):
-        """semilag.py:512-528 (trilinear trajectory sampling)."""
+        """semilag.py:512-528 (trajectory sampling at traj_order: 2 trilinear,
+        4 tricubic)."""
         lib = _load()
         x, y, z = (np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (x, y, z))
         out = np.empty(x.size)
@@ -688,7 +698,7 @@ class ChannelOracle:
     def __init__(self, nx, ny, nz, Lx, Ly, Lz, gamma, Re, U_bulk=1.0, order=6,
                  stretching_type="symmetric", top_wall_bc_type="dirichlet", n_traj_iters=2,
                  z_f=None, z_c=None, tanh_g=None, bdf2_pressure="noninc",
-                 bdf2_xy_rhs="extrap"):
+                 bdf2_xy_rhs="extrap", traj_order=2):
         # solver.py:265-281: the two BDF2 options and their checks
         if bdf2_pressure not in ("noninc", "inc"):
             raise ValueError(f"sl.bdf2_pressure must be 'noninc' or 'inc', got {bdf2_pressure}")
@@ -716,7 +726,7 @@ class ChannelOracle:
                                               self.dz_f, top_wall_bc_type)
         self.sl = SLOracle(nx, ny, nz, self.dx, self.dy, Lx, Ly, Lz, self.z_f, self.z_c,
                            gamma, stretching_type=stretching_type, order=order,
-                           n_traj_iters=n_traj_iters)
+                           n_traj_iters=n_traj_iters, traj_order=traj_order)
         if tanh_g is not None:
             self.sl.set_tanh_g(tanh_g)
         self.u = self.v = self.w = self.p = None

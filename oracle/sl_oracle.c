@@ -35,6 +35,9 @@ typedef struct {
     const double* dinv_f2;  /* (nz+1-2+1, 2) face nodes */
     const double* dinv_cP;  /* (nz+2-P+1, P) */
     const double* dinv_fP;  /* (nz+1-P+1, P) */
+    int traj_order;         /* trajectory sampling order T (2 = trilinear | 4 = tricubic) */
+    const double* dinv_cT;  /* (nz+2-T+1, T) */
+    const double* dinv_fT;  /* (nz+1-T+1, T) */
 } orc_grid;
 
 /* Node-grid spec of one component (semilag.py:282-286). */
@@ -196,14 +199,17 @@ static double gather(const orc_spec* s, const double* F, const stencil* st, int 
     return out;
 }
 
-/* semilag.py:512-528 (_sample): trajectory velocity at order 2 */
+/* semilag.py:512-528 (_sample): trajectory velocity at order traj_order
+ * (2 = trilinear, 4 = tricubic): the order-T stencil without clamp counting
+ * and the sequential gather */
 static double sample(const orc_grid* g, int comp, const double* F, double x, double y,
                      double z) {
     orc_spec s = make_spec(g, comp);
-    const double* dinv = s.centers ? g->dinv_c2 : g->dinv_f2;
+    const int T = g->traj_order;
+    const double* dinv = s.centers ? g->dinv_cT : g->dinv_fT;
     stencil st;
-    build_stencil(g, &s, dinv, x, y, clampd(z, g->z_lo, g->z_hi), 2, &st);
-    return gather(&s, F, &st, 2);
+    build_stencil(g, &s, dinv, x, y, clampd(z, g->z_lo, g->z_hi), T, &st);
+    return gather(&s, F, &st, T);
 }
 
 /* Arrival coordinates (semilag.py:292-307) of the flat interior index n. */
@@ -276,8 +282,8 @@ long orc_gather(const orc_grid* g, int comp, const double* F, const double* xd,
     return ncl;
 }
 
-/* Trilinear trajectory sample at arbitrary points (semilag.py:512-528), for
- * tests of the sampling alone. */
+/* Trajectory sample (order traj_order) at arbitrary points
+ * (semilag.py:512-528), for tests of the sampling alone. */
 void orc_sample(const orc_grid* g, int comp, const double* F, long n, const double* x,
                 const double* y, const double* z, double* out) {
 #pragma omp parallel for schedule(static)

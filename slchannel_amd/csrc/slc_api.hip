@@ -144,8 +144,12 @@ int slc::plan_create(const slc_grid_desc* d, int device, bool spectral, slc_plan
         return fail(SLC_EINVAL, "grid too small (nx=%d ny=%d nz=%d)", d->nx, d->ny, d->nz);
     if (d->order != 4 && d->order != 6)
         return fail(SLC_EINVAL, "sl.interp_order must be 4 or 6, got %d", d->order);
-    if (d->traj_order != 2)
-        return fail(SLC_EINVAL, "traj_interp_order %d not supported (trilinear = 2 only)",
+    if (d->traj_order != 2 && d->traj_order != 4)
+        return fail(SLC_EINVAL,
+                    "sl.traj_interp_order must be 2 (trilinear) or 4 (tricubic), got %d",
+                    d->traj_order);
+    if (d->nz + 1 < d->traj_order)
+        return fail(SLC_EINVAL, "nz=%d too small for trajectory order %d", d->nz,
                     d->traj_order);
     if (d->nz + 1 < d->order)
         return fail(SLC_EINVAL, "nz=%d too small for interpolation order %d", d->nz, d->order);
@@ -217,6 +221,17 @@ int slc::plan_create(const slc_grid_desc* d, int device, bool spectral, slc_plan
     UP(denom_table(zf, 2), g.dinv_f2);
     UP(denom_table(zc, P), g.dinv_cP);
     UP(denom_table(zf, P), g.dinv_fP);
+    g.traj_order = d->traj_order;
+    if (g.traj_order == 2) {
+        g.dinv_cT = g.dinv_c2;
+        g.dinv_fT = g.dinv_f2;
+    } else if (g.traj_order == P) {
+        g.dinv_cT = g.dinv_cP;
+        g.dinv_fT = g.dinv_fP;
+    } else {
+        UP(denom_table(zc, g.traj_order), g.dinv_cT);
+        UP(denom_table(zf, g.traj_order), g.dinv_fT);
+    }
     UP(dzc, p->d_dzc);
     UP(dzf, p->d_dzf);
     {
@@ -434,6 +449,12 @@ int slc_locate_z(slc_plan* p, int comp, const double* z, int64_t n, int32_t* m0_
     return SLC_OK;
 }
 
+// The LDS-tiled step kernels trace with trilinear trajectories; tricubic ones
+// (traj_interp_order 4) run the direct-load kernels.
+static bool tiled_steps(const slc_plan* p) {
+    return p->gather_shape >= 0 && p->g.traj_order == 2;
+}
+
 int slc_advect(slc_plan* p, const double* u, const double* v, const double* w, const double* um,
                const double* vm, const double* wm, double dt, double* ustar, double* vstar,
                double* wstar, int64_t* nclamp, void* stream) {
@@ -445,14 +466,14 @@ int slc_advect(slc_plan* p, const double* u, const double* v, const double* w, c
     const double* F[3] = {u, v, w};
     double* O[3] = {ustar, vstar, wstar};
     auto* ncl = reinterpret_cast<unsigned long long*>(nclamp);
-    const bool f32 = p->storage == 32 && p->gather_shape >= 0;
+    const bool f32 = p->storage == 32 && tiled_steps(p);
     if (f32) {
         Timed t(p, "to_f32", stream);
         HIP_OK(run_to_f32(p->g, p->f32n, F, S(stream)));
     }
     for (int c = 0; c < 3; ++c) {
         Timed t(p, kLabel[c][3], stream);
-        if (p->gather_shape >= 0)
+        if (tiled_steps(p))
             HIP_OK(run_tile_step(p->g, c, 0, p->at[c], V, F[c], nullptr,
                                  f32 ? p->f32n[c] : nullptr, nullptr, dt, O[c], nullptr,
                                  nullptr, nullptr, ncl, S(stream)));
@@ -472,14 +493,14 @@ int slc_bdf1_predict(slc_plan* p, const double* u, const double* v, const double
     const double* F[3] = {u, v, w};
     double* O[3] = {ustar, vstar, wstar};
     auto* ncl = reinterpret_cast<unsigned long long*>(nclamp);
-    const bool f32 = p->storage == 32 && p->gather_shape >= 0;
+    const bool f32 = p->storage == 32 && tiled_steps(p);
     if (f32) {
         Timed t(p, "to_f32", stream);
         HIP_OK(run_to_f32(p->g, p->f32n, F, S(stream)));
     }
     for (int c = 0; c < 3; ++c) {
         Timed t(p, kLabel[c][2], stream);
-        if (p->gather_shape >= 0)
+        if (tiled_steps(p))
             HIP_OK(run_tile_step(p->g, c, 1, p->at[c], V, F[c], nullptr,
                                  f32 ? p->f32n[c] : nullptr, nullptr, dt, O[c], nullptr,
                                  nullptr, nullptr, ncl, S(stream)));
@@ -501,7 +522,7 @@ int slc_bdf2_predict(slc_plan* p, const double* u, const double* v, const double
     const double* Fm[3] = {unm1, vnm1, wnm1};
     double* O[3] = {ustar, vstar, wstar};
     auto* ncl = reinterpret_cast<unsigned long long*>(nclamp);
-    if (p->gather_shape >= 0) {
+    if (tiled_steps(p)) {
         const bool f32 = p->storage == 32;
         {
             Timed t(p, "ustar", stream);

@@ -38,6 +38,9 @@ struct Grid {
     const double* dinv_f2;  // order-2, face nodes
     const double* dinv_cP;  // order-P, centre nodes  ((nz+2)-P+1, P)
     const double* dinv_fP;  // order-P, face nodes    ((nz+1)-P+1, P)
+    int traj_order;         // trajectory sampling order T: 2 trilinear, 4 tricubic
+    const double* dinv_cT;  // order-T, centre nodes  ((nz+2)-T+1, T)
+    const double* dinv_fT;  // order-T, face nodes    ((nz+1)-T+1, T)
 };
 
 // Component specs (semilag.py:282-286 node grids; solver.py:70-107 shapes).
@@ -346,11 +349,53 @@ __device__ inline double trilinear(const Grid& g, const TV& V, double xdx, doubl
     return out;
 }
 
+// Tricubic trajectory sample of component C (traj_interp_order = 4): the
+// order-4 stencil of semilag.py:471-500 (no clamp counting; zc already
+// clamped) and the sequential (i, j, k) sum of _gather_interp
+// (semilag.py:44-59), the reference's operation order.
+template <int C>
+__device__ inline double tricubic(const Grid& g, const TrajVel& V, double xdx, double ydy,
+                                  double zc, int cell) {
+    constexpr int P = 4, h = 1;
+    double sx = xdx + Comp<C>::shift_x;
+    double ix0 = floor(sx);
+    double tx = sx - ix0;
+    double sy = ydy + Comp<C>::shift_y;
+    double iy0 = floor(sy);
+    double ty = sy - iy0;
+    double wx[P], wy[P], wz[P];
+    uniform_weights<P>(tx, wx);
+    uniform_weights<P>(ty, wy);
+    int k0 = locate_from_cell<C>(g, cell, zc) - h;
+    const int NZ = Comp<C>::NZ(g);
+    if (k0 < 0) k0 = 0;
+    if (k0 > NZ - P) k0 = NZ - P;
+    z_weights<P>(znodes<C>(g), Comp<C>::centers ? g.dinv_cT : g.dinv_fT, zc, k0, wz);
+    const int fx = floor_int(ix0) - h, fy = floor_int(iy0) - h;
+    double out = 0.0;
+#pragma unroll 1
+    for (int i = 0; i < P; ++i) {  // rolled: 16 loads in flight, not 64
+        const int ix = wrap_idx(fx + i, g.nx);
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const int base = node_addr<C>(g, ix, wrap_idx(fy + j, g.ny), k0);
+            double wij = wx[i] * wy[j];
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                double term = (wij * wz[k]) * V.at(C, base + k);
+                out = (i == 0 && j == 0 && k == 0) ? term : out + term;
+            }
+        }
+    }
+    return out;
+}
+
 // Departure foot of one arrival point (semilag.py:534-552): K iterations of
 // x_m <- x_a - (dt/2) U*(x_m), z_m clamped; returns x_d = 2 x_m - x_a with z
 // unclamped.  The three trajectory samples at one point share x/dx, y/dy and
 // the face cell of z (the reference recomputes them with identical results).
-template <int DUMMY = 0>
+// T: trajectory sampling order (2 trilinear, 4 tricubic).
+template <int T = 2>
 __device__ inline void foot(const Grid& g, const TrajVel& V, double xa, double ya, double za,
                             double half_dt, double& xd, double& yd, double& zd) {
     double xm = xa, ym = ya, zm = za;
@@ -359,9 +404,16 @@ __device__ inline void foot(const Grid& g, const TrajVel& V, double xa, double y
         double xdx = xm / g.dx;
         double ydy = ym / g.dy;
         int cell = face_cell_fast(g, zc);
-        double us = trilinear<0>(g, V, xdx, ydy, zc, cell);
-        double vs = trilinear<1>(g, V, xdx, ydy, zc, cell);
-        double ws = trilinear<2>(g, V, xdx, ydy, zc, cell);
+        double us, vs, ws;
+        if constexpr (T == 4) {
+            us = tricubic<0>(g, V, xdx, ydy, zc, cell);
+            vs = tricubic<1>(g, V, xdx, ydy, zc, cell);
+            ws = tricubic<2>(g, V, xdx, ydy, zc, cell);
+        } else {
+            us = trilinear<0>(g, V, xdx, ydy, zc, cell);
+            vs = trilinear<1>(g, V, xdx, ydy, zc, cell);
+            ws = trilinear<2>(g, V, xdx, ydy, zc, cell);
+        }
         xm = xa - half_dt * us;
         ym = ya - half_dt * vs;
         zm = clampd(za - half_dt * ws, g.z_lo, g.z_hi);

@@ -602,6 +602,9 @@ int slc_ens_add(slc_ens* e, const slc_grid_desc* d, double dt, double U_bulk, in
         return fail(SLC_EINVAL, "member order %d differs from the ensemble's %d", d->order,
                     e->order);
     if (!(dt > 0.0)) return fail(SLC_EINVAL, "dt must be positive");
+    if (d->traj_order != 2)
+        return fail(SLC_EINVAL, "ensemble members trace trilinear trajectories "
+                                "(traj_interp_order 2), got %d", d->traj_order);
     if (d->nz < 3) return fail(SLC_EINVAL, "ensemble members need nz >= 3");
     if (zsolve_cols(0, d->nz) < 1) return fail(SLC_EINVAL, "nz=%d too large for the z-solve", d->nz);
     DeviceGuard dg(e->device);

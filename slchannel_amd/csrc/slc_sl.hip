@@ -35,7 +35,7 @@ __device__ inline void block_count(unsigned long long* counter, int mine) {
     if (threadIdx.x == 0 && s_cnt) atomicAdd(counter, (unsigned long long)s_cnt);
 }
 
-template <int C>
+template <int C, int T>
 __global__ __launch_bounds__(kBlock) void departure_kernel(Grid g, TrajVel V, double dt,
                                                            double* __restrict__ xd,
                                                            double* __restrict__ yd,
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(kBlock) void departure_kernel(Grid g, TrajVel V, do
     int I, J, K;
     arrival<C>(g, n, xa, ya, za, I, J, K);
     double x, y, z;
-    foot(g, V, xa, ya, za, 0.5 * dt, x, y, z);
+    foot<T>(g, V, xa, ya, za, 0.5 * dt, x, y, z);
     xd[n] = x;
     yd[n] = y;
     zd[n] = z;
@@ -147,7 +147,7 @@ __device__ inline long out_addr(const Grid& g, int I, int J, int K) {
     return ((long)I * Comp<C>::SY(g) + J) * Comp<C>::SZ(g) + K;
 }
 
-template <int C, int P>
+template <int C, int P, int T>
 __global__ __launch_bounds__(kBlock) void advect_kernel(Grid g, TrajVel V,
                                                         const double* __restrict__ F, double dt,
                                                         double* __restrict__ out, long N,
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void advect_kernel(Grid g, TrajVel V,
         int I, J, K;
         arrival<C>(g, n, xa, ya, za, I, J, K);
         double x, y, z;
-        foot(g, V, xa, ya, za, 0.5 * dt, x, y, z);
+        foot<T>(g, V, xa, ya, za, 0.5 * dt, x, y, z);
         clamped = (z < g.z_lo || z > g.z_hi) ? 1 : 0;
         z = clampd(z, g.z_lo, g.z_hi);
         Stencil<C, P> s;
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(kBlock) void advect_kernel(Grid g, TrajVel V,
 }
 
 // BDF1 bootstrap predictor: ustar = advect(V^n; U* = V^n, dt) + dt * R(V^n)
-template <int C, int P>
+template <int C, int P, int T>
 __global__ __launch_bounds__(kBlock) void bdf1_kernel(Grid g, TrajVel V,
                                                       const double* __restrict__ F, double dt,
                                                       double dx2, double dy2,
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void bdf1_kernel(Grid g, TrajVel V,
         int I, J, K;
         arrival<C>(g, n, xa, ya, za, I, J, K);
         double x, y, z;
-        foot(g, V, xa, ya, za, 0.5 * dt, x, y, z);
+        foot<T>(g, V, xa, ya, za, 0.5 * dt, x, y, z);
         clamped = (z < g.z_lo || z > g.z_hi) ? 1 : 0;
         z = clampd(z, g.z_lo, g.z_hi);
         Stencil<C, P> s;
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(kBlock) void bdf1_kernel(Grid g, TrajVel V,
 // Two passes over the arrival points keep each pass at one order-P gather:
 //   far:  out <- V^{n-1} at the foot of depth 2dt          (solver.py:872-882)
 //   near: out <- 4/3 V^n(foot dt) - 1/3 out + dt_eff Rh    (solver.py:885-903)
-template <int C, int P>
+template <int C, int P, int T>
 __global__ __launch_bounds__(kBlock) void bdf2_far_kernel(Grid g, TrajVel V,
                                                           const double* __restrict__ Fm,
                                                           double dt, double* __restrict__ out,
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void bdf2_far_kernel(Grid g, TrajVel V,
         int I, J, K;
         arrival<C>(g, n, xa, ya, za, I, J, K);
         double x, y, z;
-        foot(g, V, xa, ya, za, 0.5 * (2.0 * dt), x, y, z);
+        foot<T>(g, V, xa, ya, za, 0.5 * (2.0 * dt), x, y, z);
         clamped = (z < g.z_lo || z > g.z_hi) ? 1 : 0;
         z = clampd(z, g.z_lo, g.z_hi);
         Stencil<C, P> s;
@@ -225,7 +225,7 @@ __global__ __launch_bounds__(kBlock) void bdf2_far_kernel(Grid g, TrajVel V,
     if (nclamp) block_count(nclamp, clamped);
 }
 
-template <int C, int P>
+template <int C, int P, int T>
 __global__ __launch_bounds__(kBlock) void bdf2_near_kernel(Grid g, TrajVel V,
                                                            const double* __restrict__ Fn,
                                                            const double* __restrict__ Fm,
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void bdf2_near_kernel(Grid g, TrajVel V,
         int I, J, K;
         arrival<C>(g, n, xa, ya, za, I, J, K);
         double x, y, z;
-        foot(g, V, xa, ya, za, 0.5 * dt, x, y, z);
+        foot<T>(g, V, xa, ya, za, 0.5 * dt, x, y, z);
         clamped = (z < g.z_lo || z > g.z_hi) ? 1 : 0;
         z = clampd(z, g.z_lo, g.z_hi);
         Stencil<C, P> s;
@@ -267,6 +267,19 @@ static inline long n_arrival(const Grid& g, int comp) {
 
 static inline dim3 grid_for(long N) { return dim3((unsigned)((N + kBlock - 1) / kBlock)); }
 
+// Launch KER<C, P, T> for the plan's field order P and trajectory order T.
+#define SLC_LAUNCH_PT(KER, ...)                                                    \
+    do {                                                                           \
+        if (g.order == 4 && g.traj_order == 4)                                     \
+            hipLaunchKernelGGL((KER<C, 4, 4>), __VA_ARGS__);                       \
+        else if (g.order == 4)                                                     \
+            hipLaunchKernelGGL((KER<C, 4, 2>), __VA_ARGS__);                       \
+        else if (g.traj_order == 4)                                                \
+            hipLaunchKernelGGL((KER<C, 6, 4>), __VA_ARGS__);                       \
+        else                                                                       \
+            hipLaunchKernelGGL((KER<C, 6, 2>), __VA_ARGS__);                       \
+    } while (0)
+
 #define SLC_DISPATCH_C(comp, FN, ...) \
     switch (comp) {                     \
         case 0: FN<0>(__VA_ARGS__); break; \
@@ -279,8 +292,12 @@ static void launch_departure(const Grid& g, const TrajVel& V, double dt, double*
                              double* zd, hipStream_t s) {
     long N = n_arrival(g, C);
     if (N <= 0) return;
-    hipLaunchKernelGGL(departure_kernel<C>, grid_for(N), dim3(kBlock), 0, s, g, V, dt, xd, yd, zd,
-                       N);
+    if (g.traj_order == 4)
+        hipLaunchKernelGGL((departure_kernel<C, 4>), grid_for(N), dim3(kBlock), 0, s, g, V, dt, xd,
+                           yd, zd, N);
+    else
+        hipLaunchKernelGGL((departure_kernel<C, 2>), grid_for(N), dim3(kBlock), 0, s, g, V, dt, xd,
+                           yd, zd, N);
 }
 
 hipError_t run_departure(const Grid& g, int comp, const TrajVel& V, double dt, double* xd,
@@ -341,12 +358,7 @@ static void launch_advect(const Grid& g, const TrajVel& V, const double* F, doub
                           double* out, unsigned long long* ncl, hipStream_t s) {
     long N = n_arrival(g, C);
     if (N <= 0) return;
-    if (g.order == 4)
-        hipLaunchKernelGGL((advect_kernel<C, 4>), grid_for(N), dim3(kBlock), 0, s, g, V, F, dt,
-                           out, N, ncl);
-    else
-        hipLaunchKernelGGL((advect_kernel<C, 6>), grid_for(N), dim3(kBlock), 0, s, g, V, F, dt,
-                           out, N, ncl);
+    SLC_LAUNCH_PT(advect_kernel, grid_for(N), dim3(kBlock), 0, s, g, V, F, dt, out, N, ncl);
 }
 
 hipError_t run_advect(const Grid& g, int comp, const TrajVel& V, const double* F, double dt,
@@ -360,12 +372,8 @@ static void launch_bdf1(const Grid& g, const TrajVel& V, const double* F, double
                         double dy2, double* out, unsigned long long* ncl, hipStream_t s) {
     long N = n_arrival(g, C);
     if (N <= 0) return;
-    if (g.order == 4)
-        hipLaunchKernelGGL((bdf1_kernel<C, 4>), grid_for(N), dim3(kBlock), 0, s, g, V, F, dt, dx2,
-                           dy2, out, N, ncl);
-    else
-        hipLaunchKernelGGL((bdf1_kernel<C, 6>), grid_for(N), dim3(kBlock), 0, s, g, V, F, dt, dx2,
-                           dy2, out, N, ncl);
+    SLC_LAUNCH_PT(bdf1_kernel, grid_for(N), dim3(kBlock), 0, s, g, V, F, dt, dx2, dy2, out, N,
+                  ncl);
 }
 
 hipError_t run_bdf1(const Grid& g, int comp, const TrajVel& V, const double* F, double dt,
@@ -383,19 +391,11 @@ static void launch_bdf2(const Grid& g, int pass, const TrajVel& V, const double*
     long N = n_arrival(g, C);
     if (N <= 0) return;
     if (pass == 0) {
-        if (g.order == 4)
-            hipLaunchKernelGGL((bdf2_far_kernel<C, 4>), grid_for(N), dim3(kBlock), 0, s, g, V, Fm,
-                               dt, out, N, ncl);
-        else
-            hipLaunchKernelGGL((bdf2_far_kernel<C, 6>), grid_for(N), dim3(kBlock), 0, s, g, V, Fm,
-                               dt, out, N, ncl);
+        SLC_LAUNCH_PT(bdf2_far_kernel, grid_for(N), dim3(kBlock), 0, s, g, V, Fm, dt, out, N,
+                      ncl);
     } else {
-        if (g.order == 4)
-            hipLaunchKernelGGL((bdf2_near_kernel<C, 4>), grid_for(N), dim3(kBlock), 0, s, g, V, Fn,
-                               Fm, dt, dt_eff, dx2, dy2, out, N, ncl);
-        else
-            hipLaunchKernelGGL((bdf2_near_kernel<C, 6>), grid_for(N), dim3(kBlock), 0, s, g, V, Fn,
-                               Fm, dt, dt_eff, dx2, dy2, out, N, ncl);
+        SLC_LAUNCH_PT(bdf2_near_kernel, grid_for(N), dim3(kBlock), 0, s, g, V, Fn, Fm, dt,
+                      dt_eff, dx2, dy2, out, N, ncl);
     }
 }
 

@@ -29,6 +29,9 @@ def _config_grid(cfg):
     g, dom, flow = cfg["grid"], cfg["domain"], cfg["flow"]
     sl = cfg.get("sl", {})
     bc = cfg.get("boundary_conditions", {})
+    if int(sl.get("traj_interp_order", 2)) != 2:
+        raise ValueError("ensemble members trace trilinear trajectories (traj_interp_order 2); "
+                         "tricubic ones are a single-run option of SLChannelFlow")
     if sl.get("bdf2_pressure", "noninc") != "noninc" or sl.get("bdf2_xy_rhs", "extrap") != "extrap":
         raise ValueError("ensemble members run the production BDF2 options (bdf2_pressure "
                          "'noninc', bdf2_xy_rhs 'extrap'); 'inc' / 'lagged' are single-run "

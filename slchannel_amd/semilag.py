@@ -85,8 +85,6 @@ class SLAdvector:
                 f"sl.interp_dtype must be 'fp64' or 'fp32_accum64', got {interp_dtype}")
         if nz + 1 < order:
             raise ValueError(f"nz={nz} too small for interpolation order {order}")
-        if traj_order != 2:
-            raise ValueError("the HIP layer implements trilinear trajectories (traj_order=2)")
         if not env.USE_HIP:
             raise RuntimeError("SLCHANNEL_HIP=0: the HIP semi-Lagrangian layer is disabled")
         self.nx, self.ny, self.nz = nx, ny, nz

@@ -22,7 +22,9 @@ functions whose results no closed form or invariant fixes bit-for-bit:
 * ``step_*.npz``     SLChannelFlow.step_sl_bdf2 (solver.py:813-1018), BDF1
                      bootstrap included, from a seeded projected state; two
                      cases with the non-default BDF2 options (bdf2_pressure
-                     'inc', bdf2_xy_rhs 'lagged', solver.py:256-281).
+                     'inc', bdf2_xy_rhs 'lagged', solver.py:256-281);
+* ``*_traj4.npz``    advect and step with tricubic trajectory sampling
+                     (traj_interp_order 4, semilag.py:179-209, 512-528).
 
 Inputs are generated here with numpy (seeded) and stored next to the outputs,
 so the fixtures are self-contained: data only, no reference code.
@@ -115,7 +117,7 @@ def gen_grid(ref, name, gamma, nz, Lz, stretching):
     np.savez_compressed(os.path.join(HERE, f"grid_{name}.npz"), **d)
 
 
-def gen_advect(ref, name, order, stretching="symmetric", clamp=False, seed=1):
+def gen_advect(ref, name, order, stretching="symmetric", clamp=False, seed=1, traj_order=2):
     torch, semilag, solver, utils, projection, operators = ref
     nx, ny, nz = 12, 10, 16
     Lx, Ly, Lz, gamma = 4 * math.pi, 2 * math.pi, 2.0, 1.5
@@ -138,12 +140,12 @@ def gen_advect(ref, name, order, stretching="symmetric", clamp=False, seed=1):
     dt = 1.6 if not clamp else 0.9
 
     adv = semilag.SLAdvector(nx, ny, nz, dx, dy, Lx, Ly, Lz, z_f, z_c, gamma,
-                             stretching_type=stretching, order=order)
+                             stretching_type=stretching, order=order, traj_order=traj_order)
     ts = [torch.from_numpy(a.copy()) for a in (u, v, w, um, vm, wm)]
     ustar, vstar, wstar = adv.advect(*ts, torch.tensor(dt))
     d = {
         "nx": nx, "ny": ny, "nz": nz, "Lx": Lx, "Ly": Ly, "Lz": Lz, "gamma": gamma,
-        "stretching": stretching, "order": order, "dt": dt,
+        "stretching": stretching, "order": order, "dt": dt, "traj_order": traj_order,
         "z_f": zf, "z_c": zc,
         "u": u, "v": v, "w": w, "um": um, "vm": vm, "wm": wm,
         "ustar": ustar.numpy().copy(), "vstar": vstar.numpy().copy(),
@@ -262,6 +264,7 @@ def gen_step(ref, name, order, stretching="symmetric", top="dirichlet", n_steps=
             "Re": Re, "dt": dt, "order": order, "stretching": stretching, "top": top,
             "bdf2_pressure": (sl_extra or {}).get("bdf2_pressure", "noninc"),
             "bdf2_xy_rhs": (sl_extra or {}).get("bdf2_xy_rhs", "extrap"),
+            "traj_order": (sl_extra or {}).get("traj_interp_order", 2),
             "z_f": zf, "z_c": zc,
             "u0": tu.numpy().copy(), "v0": tv.numpy().copy(), "w0": tw.numpy().copy(),
             "p0": p.numpy().copy(),
@@ -304,6 +307,9 @@ def main():
                                          sl_extra={"bdf2_pressure": "inc"})),
         ("step_o6_lagged", lambda: gen_step(ref, "o6_lagged", 6, seed=7,
                                             sl_extra={"bdf2_xy_rhs": "lagged"})),
+        ("advect_o6_traj4", lambda: gen_advect(ref, "o6_traj4", 6, seed=8, traj_order=4)),
+        ("step_o4_traj4", lambda: gen_step(ref, "o4_traj4", 4, seed=9,
+                                           sl_extra={"traj_interp_order": 4})),
     ]
     for name, make in cases:
         if not only or name in only:

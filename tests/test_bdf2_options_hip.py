@@ -1,4 +1,6 @@
-"""The non-default BDF2 options on the HIP layer (solver.py:256-281, 846-1012):
+"""The non-default step options on the HIP layer (solver.py:253-281, 846-1012;
+semilag.py:179-209): sl.traj_interp_order 4 (tricubic trajectory samples, the
+direct-load kernels), and the BDF2 options
 sl.bdf2_xy_rhs 'lagged' (the predictor kernels take R^n) and
 sl.bdf2_pressure 'inc' (p_ext = 2 p^n - p^{n-1} through slc_pressure_combine,
 -dt_eff grad p_ext added to the predictor through slc_project before the
@@ -39,8 +41,9 @@ def test_solver_option_checks(check):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("opts", [{"bdf2_xy_rhs": "lagged"}, {"bdf2_pressure": "inc"},
-                                  {"bdf2_pressure": "inc", "bdf2_xy_rhs": "lagged"}],
-                         ids=["lagged", "inc", "inc+lagged"])
+                                  {"bdf2_pressure": "inc", "bdf2_xy_rhs": "lagged"},
+                                  {"traj_interp_order": 4}],
+                         ids=["lagged", "inc", "inc+lagged", "traj4"])
 def test_options_vs_oracle(check, opts):
     from slchannel_amd import SLChannelFlow
     from slchannel_amd.diagnostics import cfl_dt
@@ -49,7 +52,9 @@ def test_options_vs_oracle(check, opts):
     flow.initialize(seed=1003, urms=0.1, kmax=6.0)
     co = orc.ChannelOracle(flow.nx, flow.ny, flow.nz, flow.Lx, flow.Ly, flow.Lz, flow.gamma,
                            2800.0, order=4, z_f=flow.z_f.numpy(), z_c=flow.z_c.numpy(),
-                           tanh_g=flow.plan.tanh_g, **opts)
+                           tanh_g=flow.plan.tanh_g,
+                           traj_order=opts.get("traj_interp_order", 2),
+                           **{k: v for k, v in opts.items() if k.startswith("bdf2_")})
     co.set_fields(*(t.cpu().numpy() for t in (flow.u, flow.v, flow.w, flow.p)))
     dt = cfl_dt(flow, 3.0)
     worst_v, worst_p = 0.0, 0.0
@@ -69,3 +74,13 @@ def test_options_vs_oracle(check, opts):
         phi = flow._phi
         d = _rel(flow.p, phi.cpu().numpy())
         check("p = p_ext + phi differs from phi", d > 1e-6, f"rel {d:.1e}")
+
+
+def test_ensemble_rejects_single_run_options(check):
+    from slchannel_amd.ensemble import _config_grid
+
+    for sl, msg in (({"bdf2_pressure": "inc"}, "noninc"), ({"bdf2_xy_rhs": "lagged"}, "extrap"),
+                    ({"traj_interp_order": 4}, "trilinear")):
+        with pytest.raises(ValueError, match=msg):
+            _config_grid(_cfg(**sl))
+    check("ensemble members keep the production options", True, "")

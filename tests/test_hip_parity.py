@@ -39,6 +39,7 @@ def make_plan(g, order=None, nu=0.0, top=None):
                    g["z_c"], float(g["gamma"]),
                    stretching_type=str(g.get("stretching", "symmetric")),
                    order=int(order if order is not None else g["order"]),
+                   traj_order=int(g["traj_order"]) if "traj_order" in g else 2,
                    top_wall_bc_type=top or str(g.get("top", "dirichlet")), nu=nu)
 
 
@@ -69,7 +70,7 @@ def test_ghost_fill_bitexact(check, top):
         check(f"bc {nm} ({top})", np.array_equal(a.cpu().numpy(), b), "bit-exact")
 
 
-@pytest.mark.parametrize("name", ["o4", "o6", "o6_bottom", "o4_clamp"])
+@pytest.mark.parametrize("name", ["o4", "o6", "o6_bottom", "o4_clamp", "o6_traj4"])
 def test_feet_stencils_advect_vs_reference(check, golden, name):
     g = golden("advect_" + name)
     plan = make_plan(g)
@@ -255,12 +256,13 @@ def _flow_from_golden(g):
         "time": {"dt": float(g["dt"])},
         "sl": {"interp_order": int(g["order"]), "n_traj_iters": 2,
                "bdf2_pressure": str(g["bdf2_pressure"]) if "bdf2_pressure" in g else "noninc",
-               "bdf2_xy_rhs": str(g["bdf2_xy_rhs"]) if "bdf2_xy_rhs" in g else "extrap"},
+               "bdf2_xy_rhs": str(g["bdf2_xy_rhs"]) if "bdf2_xy_rhs" in g else "extrap",
+               "traj_interp_order": int(g["traj_order"]) if "traj_order" in g else 2},
     }
     return SLChannelFlow(config=cfg, z_tables=(g["z_f"], g["z_c"]))
 
 
-@pytest.mark.parametrize("name", ["o6", "o4_bottom_neumann", "o4_inc", "o6_lagged"])
+@pytest.mark.parametrize("name", ["o6", "o4_bottom_neumann", "o4_inc", "o6_lagged", "o4_traj4"])
 def test_step_vs_reference(check, golden, name):
     g = golden("step_" + name)
     flow = _flow_from_golden(g)

@@ -62,10 +62,11 @@ def _sl_from(g):
     nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
     Lx, Ly, Lz, gamma = float(g["Lx"]), float(g["Ly"]), float(g["Lz"]), float(g["gamma"])
     return orc.SLOracle(nx, ny, nz, Lx / nx, Ly / ny, Lx, Ly, Lz, g["z_f"], g["z_c"], gamma,
-                        stretching_type=str(g["stretching"]), order=int(g["order"]))
+                        stretching_type=str(g["stretching"]), order=int(g["order"]),
+                        traj_order=int(g["traj_order"]) if "traj_order" in g else 2)
 
 
-@pytest.mark.parametrize("name", ["o4", "o6", "o6_bottom", "o4_clamp"])
+@pytest.mark.parametrize("name", ["o4", "o6", "o6_bottom", "o4_clamp", "o6_traj4"])
 def test_advect_vs_reference(check, golden, name):
     g = golden("advect_" + name)
     sl = _sl_from(g)
@@ -125,11 +126,13 @@ def test_solves_vs_reference(check, golden, tag):
 
 def _bdf2_options(g):
     """The fixture's BDF2 options (the defaults for files without them)."""
-    return {k: str(g[k]) if k in g else d
+    opts = {k: str(g[k]) if k in g else d
             for k, d in (("bdf2_pressure", "noninc"), ("bdf2_xy_rhs", "extrap"))}
+    opts["traj_order"] = int(g["traj_order"]) if "traj_order" in g else 2
+    return opts
 
 
-@pytest.mark.parametrize("name", ["o6", "o4_bottom_neumann", "o4_inc", "o6_lagged"])
+@pytest.mark.parametrize("name", ["o6", "o4_bottom_neumann", "o4_inc", "o6_lagged", "o4_traj4"])
 def test_step_vs_reference(check, golden, name):
     g = golden("step_" + name)
     nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
