@@ -50,6 +50,8 @@ def parse():
                     help="bits of the gathered field copies: 64 (default) or 32 "
                          "(FP32-storage variant; adds an FP64 twin run and reports the "
                          "per-field difference)")
+    ap.add_argument("--traj-order", type=int, default=2, choices=(2, 4),
+                    help="trajectory sampling order (4: tricubic, the direct-load kernels)")
     ap.add_argument("--bdf2-split", action="store_true",
                     help="far pass, then near pass, instead of the fused BDF2 pass "
                          "(same doubles; for profiling the two passes)")
@@ -109,7 +111,8 @@ def main():
         "domain": {"Lx": 4 * math.pi, "Ly": 2 * math.pi, "Lz": 2.0},
         "flow": {"Re": Re_b, "U_bulk": 1.0, "gamma": 1.5},
         "sl": {"interp_order": args.order,
-               "interp_dtype": "fp32_accum64" if args.storage == 32 else "fp64"},
+               "interp_dtype": "fp32_accum64" if args.storage == 32 else "fp64",
+               "traj_interp_order": args.traj_order},
     }
     flow = SLChannelFlow(config=cfg, device="cuda:0")
     flow.initialize(seed=2000, urms=0.15, kmin=1.0, kmax=32.0, spectrum="k-5/3")
@@ -181,8 +184,9 @@ def main():
 
     kernels = {k: {"launches": c, "avg_ms": round(t / c, 4)} for k, (c, t) in sorted(prof.items())}
     line = {
-        "metric": "steps/s (SL-BDF2 step, 256^3 triquintic fp64)" if args.storage == 64 else
-                  "steps/s (SL-BDF2 step, 256^3 triquintic, FP32-storage gathers)",
+        "metric": ("steps/s (SL-BDF2 step, 256^3 triquintic fp64)" if args.storage == 64 else
+                   "steps/s (SL-BDF2 step, 256^3 triquintic, FP32-storage gathers)")
+                  + ("" if args.traj_order == 2 else ", tricubic trajectories"),
         "value": round(args.steps / (t1 - t0), 3),
         "unit": "steps/s",
         "n_gpus": 1,
@@ -197,12 +201,13 @@ def main():
         "config": {"workload": f"SL-BDF2 channel {n}x{n}x{nz} order {args.order} "
                                f"Re_b {Re_b} dt+ {dtp}",
                    "nx": n, "ny": n, "nz": nz, "interp_order": args.order, "dt": dt,
-                   "storage_bits": args.storage},
+                   "storage_bits": args.storage, "traj_interp_order": args.traj_order},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBPS, 4), "traffic": traffic,
                      "bytes_per_launch": bytes_far, "avg_ms": round(avg_ms, 4),
-                     "fp64_check": fp64_check(lab, avg_ms, N_u, args.order)},
+                     "fp64_check": fp64_check(lab, avg_ms, N_u, args.order)
+                     if args.traj_order == 2 else None},
         "gather_GBps": round(gather_gbps, 1),
         "gather_frac_hbm": round(gather_gbps / PEAK_HBM_GBPS, 4),
         "max_div": div,
