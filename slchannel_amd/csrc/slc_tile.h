@@ -176,16 +176,20 @@ __device__ inline bool stage_box(const Grid& g, const T* __restrict__ S, const S
                                       wrap_idx(b.ymin + y, g.ny), b.kmin);
         }
         __syncthreads();
-        // flat element loop (z fastest: coalesced rows); the quotient e/BZ
-        // comes from a float reciprocal, made exact by one integer
-        // correction each way (the estimate is within 1 of the quotient)
-        const float rbz = 1.0f / (float)BZ;
+        // flat element loop (z fastest: coalesced rows), e = row BZ + zz
+        // carried from one iteration to the next: e grows by kSB = dq BZ + dr
+        // with dr < BZ, so zz + dr < 2 BZ needs at most one carry
+        const int dq = kSB / BZ, dr = kSB - dq * BZ;
+        int row = (int)threadIdx.x / BZ;
+        int zz = (int)threadIdx.x - row * BZ;
         for (int e = threadIdx.x; e < nbox; e += kSB) {
-            int row = (int)(((float)e + 0.5f) * rbz);
-            row += (row + 1) * BZ <= e ? 1 : 0;
-            row -= row * BZ > e ? 1 : 0;
-            const int zz = e - row * BZ;
             lds[e] = S[rowbase[row] + zz];
+            row += dq;
+            zz += dr;
+            if (zz >= BZ) {
+                zz -= BZ;
+                ++row;
+            }
         }
     }
     __syncthreads();
