@@ -37,7 +37,11 @@
 
 namespace slc {
 
-constexpr int kSB = 256;  // threads per workgroup
+// threads per workgroup (compile-time overridable for tile sweeps)
+#ifndef SLC_TILE_NT
+#define SLC_TILE_NT 256
+#endif
+constexpr int kSB = SLC_TILE_NT;
 
 enum { MODE_ADVECT = 0, MODE_BDF1 = 1, MODE_FAR = 2, MODE_NEAR = 3, MODE_GATHER = 4,
        MODE_BDF2 = 5 };
