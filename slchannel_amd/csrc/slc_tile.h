@@ -58,21 +58,26 @@ struct SBox {
     int xmin, xmax, ymin, ymax, kmin, kmax;
 };
 
-__device__ inline int s_wave_min(int v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        int w = __shfl_xor(v, o);
-        v = w < v ? w : v;
-    }
-    return v;
+// Wavefront min / max with DPP lane moves (registers only): swaps within
+// quads, half-row and row mirrors give every lane its 16-lane row's value;
+// row broadcasts of lanes 15 and 31 fold the rows into lane 63, read back as
+// a uniform value.  A lane with no DPP source keeps its own value, which min
+// and max ignore.
+template <bool MIN>
+__device__ inline int s_wave_reduce(int v) {
+    auto op = [](int a, int b) { return MIN ? (b < a ? b : a) : (b > a ? b : a); };
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));  // row_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));  // row_bcast:15
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));  // row_bcast:31
+    return __builtin_amdgcn_readlane(v, 63);
 }
 
-__device__ inline int s_wave_max(int v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        int w = __shfl_xor(v, o);
-        v = w > v ? w : v;
-    }
-    return v;
-}
+__device__ inline int s_wave_min(int v) { return s_wave_reduce<true>(v); }
+
+__device__ inline int s_wave_max(int v) { return s_wave_reduce<false>(v); }
 
 // Workgroup union of the lanes' boxes, left in sh[0..5] (xmin, xmax, ymin,
 // ymax, kmin, kmax); ends with a barrier.
