@@ -158,10 +158,17 @@ __device__ inline int face_cell_fast(const Grid& g, double z) {
     }
     int c = (int)floorf(fminf(fmaxf(kfe, 0.0f), (float)(nz - 1)));
     const double* zf = g.z_f;
-    int guard = 0;
-    while (c > 0 && z < zf[c] && guard < 8) { --c; ++guard; }
-    while (c < nz - 1 && z >= zf[c + 1] && guard < 16) { ++c; ++guard; }
-    const double lo = zf[c], hi = zf[c + 1];
+    // both faces of the estimate at once (c + 1 <= nz is inside z_f); the
+    // refinement loops only run when the estimate does not bracket z, which
+    // is exactly when they would take a step
+    double lo = zf[c], hi = zf[c + 1];
+    if (!((c == 0 || z >= lo) && (c == nz - 1 || z < hi))) {
+        int guard = 0;
+        while (c > 0 && z < zf[c] && guard < 8) { --c; ++guard; }
+        while (c < nz - 1 && z >= zf[c + 1] && guard < 16) { ++c; ++guard; }
+        lo = zf[c];
+        hi = zf[c + 1];
+    }
     const double band = 1e-10 * (hi - lo);
     const bool bracketed = (c == 0 || z >= lo) && (c == nz - 1 || z < hi);
     if (!bracketed || z - lo <= band || hi - z <= band) return face_cell(g, z);
