@@ -46,7 +46,9 @@ typedef struct {
     double tanh_g;         /* tanh(gamma) as the host computed it (semilag.py:240-242) */
     int stretching;        /* 0 = 'symmetric', 1 = 'bottom' (semilag.py:210-213) */
     int order;             /* 4 | 6 (semilag.py:204-205) */
-    int traj_order;        /* 2 (trilinear trajectory sampling; semilag.py:206-209) */
+    int traj_order;        /* trajectory sampling: 2 trilinear (the LDS-tiled step
+                              kernels), 4 tricubic (direct-load kernels);
+                              semilag.py:179-209 */
     int n_traj_iters;      /* midpoint fixed-point iterations (semilag.py:543) */
     int top_wall;          /* 0 = 'dirichlet' (no-slip), 1 = 'neumann' (free-slip) */
     double nu;             /* 1/Re (solver.py:209) */
@@ -79,14 +81,21 @@ int slc_plan_workspace_bytes(const slc_plan* plan, int64_t* out);
 int slc_apply_bc(slc_plan* plan, double* u, double* v, double* w, void* stream);
 
 /* Departure feet of every arrival point of component `comp` by the iterated
- * midpoint rule with trilinear sampling of the trajectory velocity
- * (um, vm, wm: ghost-shaped).  Outputs are flat arrays of nx*ny*nzi doubles
- * (nzi = nz for u, v and nz-1 for w, z fastest); x, y unwrapped, z NOT
- * clamped.  Replaces SLAdvector.departure_coords (semilag.py:534-552) /
- * TritonSL.departure (semilag_triton.py:384-420). */
+ * midpoint rule, sampling the trajectory velocity (um, vm, wm) at the plan's
+ * traj_order (2 trilinear, 4 tricubic).  layout 0: um, vm, wm are the
+ * ghost-shaped fields; layout 1: the packed node buffers (nx, ny, NZ) that
+ * SLAdvector._fill writes into SLAdvector.mbuf (semilag.py:387-394, 604-606;
+ * NZ = nz+2 for u, v and nz+1 for w).  Outputs are flat arrays of nx*ny*nzi
+ * doubles (nzi = nz for u, v and nz-1 for w, z fastest); x, y unwrapped.
+ * nclamp NULL: z is NOT clamped (SLAdvector.departure_coords,
+ * semilag.py:534-552).  nclamp non-NULL (a device int64): z is clamped into
+ * [z_lo, z_hi] and the number of clamped feet is added to *nclamp
+ * (TritonSL.departure, semilag_triton.py:384-420, 262-269); a later
+ * slc_gather then needs no counter.
+ * Replaces SLAdvector.departure_coords / TritonSL.departure. */
 int slc_departure(slc_plan* plan, int comp, const double* um, const double* vm,
-                  const double* wm, double dt, double* xd, double* yd, double* zd,
-                  void* stream);
+                  const double* wm, int layout, double dt, double* xd, double* yd, double* zd,
+                  int64_t* nclamp, void* stream);
 
 /* Clamp the feet into [z_lo, z_hi] (adding the number clamped to *nclamp, a
  * device int64, if non-NULL), build the order-P stencils and interpolate
@@ -214,7 +223,7 @@ int slc_bulk_force(slc_plan* plan, double* u, double* v, double* w, double U_bul
 
 /* Options.
  * SLC_OPT_GATHER_TILE (no counterpart in the reference): -1 = direct
- * global-load kernels; 0 = fused LDS-tiled step kernels (4x4x32 arrival
+ * global-load kernels; 0 = fused LDS-tiled step kernels (4x4x16 arrival
  * points per tile, default); 1..3 = older standalone tiled gathers for
  * slc_gather (1: 2x8x32, 2: 2x4x64, 3: 4x4x32).
  * SLC_OPT_FIELD_STORAGE: 64 (default) or 32.  At 32 the gathers of

@@ -69,7 +69,7 @@ def load():
         "slc_plan_destroy": ([vp], i),
         "slc_plan_workspace_bytes": ([vp, ctypes.POINTER(ctypes.c_int64)], i),
         "slc_apply_bc": ([vp, vp, vp, vp, vp], i),
-        "slc_departure": ([vp, i, vp, vp, vp, d, vp, vp, vp, vp], i),
+        "slc_departure": ([vp, i, vp, vp, vp, i, d, vp, vp, vp, i64p, vp], i),
         "slc_gather": ([vp, i, vp, i, vp, vp, vp, vp, i64p, vp], i),
         "slc_stencil": ([vp, i, vp, vp, vp, vp, vp, vp, i64p, vp], i),
         "slc_locate_z": ([vp, i, vp, ctypes.c_int64, vp, vp, vp], i),

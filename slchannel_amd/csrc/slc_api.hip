@@ -387,14 +387,17 @@ int slc_apply_bc(slc_plan* p, double* u, double* v, double* w, void* stream) {
 }
 
 int slc_departure(slc_plan* p, int comp, const double* um, const double* vm, const double* wm,
-                  double dt, double* xd, double* yd, double* zd, void* stream) {
+                  int layout, double dt, double* xd, double* yd, double* zd, int64_t* nclamp,
+                  void* stream) {
     if (int rc = check_plan(p)) return rc;
     if (comp < 0 || comp > 2) return fail(SLC_EINVAL, "comp must be 0, 1 or 2");
+    if (layout != 0 && layout != 1) return fail(SLC_EINVAL, "layout must be 0 or 1");
     if (!um || !vm || !wm || !xd || !yd || !zd) return fail(SLC_EINVAL, "null array");
     DeviceGuard dg(p->device);
     TrajVel V{{um, vm, wm}, {nullptr, nullptr, nullptr}};
     Timed t(p, kLabel[comp][5], stream);
-    HIP_OK(run_departure(p->g, comp, V, dt, xd, yd, zd, S(stream)));
+    HIP_OK(run_departure(p->g, comp, V, layout, dt, xd, yd, zd,
+                         reinterpret_cast<unsigned long long*>(nclamp), S(stream)));
     return SLC_OK;
 }
 

@@ -252,8 +252,11 @@ __device__ inline void z_weights(const double* zn, const double* dinv, double z,
 // (32-bit: plans keep every field below 2^31 elements.  Signed on purpose:
 // with no wrap-around the compiler proves base+k, base+k+1 adjacent and
 // merges their loads into one 16-byte load.)
-template <int C>
+// PACKED: the address inside a packed node buffer (nx, ny, NZ) of the
+// reference's _fill (semilag.py:387-394) instead.
+template <int C, bool PACKED = false>
 __device__ inline int node_addr(const Grid& g, int ib, int jb, int k) {
+    if (PACKED) return (ib * g.ny + jb) * Comp<C>::NZ(g) + k;
     return ((ib + Comp<C>::ox) * (int)Comp<C>::SY(g) + (jb + Comp<C>::oy)) *
                (int)Comp<C>::SZ(g) + k;
 }
@@ -291,6 +294,7 @@ __device__ inline void arrival(const Grid& g, long n, double& xa, double& ya, do
 // Trajectory velocity accessor: either a stored field or U* = 2a - b formed
 // on the fly (solver.py:866-868 forms the same single rounding).
 struct TrajVel {
+    static constexpr bool packed = false;
     const double* a[3];
     const double* b[3];  // null => U* = a
     __device__ inline double at(int c, long idx) const {
@@ -300,6 +304,16 @@ struct TrajVel {
 
 // Trajectory velocity that is always a stored field (U* precomputed, or V^n).
 struct TrajA {
+    static constexpr bool packed = false;
+    const double* a[3];
+    __device__ inline double at(int c, long idx) const { return a[c][idx]; }
+};
+
+// Trajectory velocity read from the reference's packed node buffers
+// (SLAdvector.mbuf, (nx, ny, NZ) per component, semilag.py:363-372, 604-606),
+// the layout TritonSL.departure samples (semilag_triton.py:384-420).
+struct TrajNodes {
+    static constexpr bool packed = true;
     const double* a[3];
     __device__ inline double at(int c, long idx) const { return a[c][idx]; }
 };
@@ -343,7 +357,7 @@ __device__ inline double trilinear(const Grid& g, const TV& V, double xdx, doubl
     for (int i = 0; i < 2; ++i) {
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            const int base = node_addr<C>(g, ix[i], iy[j], k0);
+            const int base = node_addr<C, TV::packed>(g, ix[i], iy[j], k0);
             double wij = wx[i] * wy[j];
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
@@ -360,8 +374,8 @@ __device__ inline double trilinear(const Grid& g, const TV& V, double xdx, doubl
 // order-4 stencil of semilag.py:471-500 (no clamp counting; zc already
 // clamped) and the sequential (i, j, k) sum of _gather_interp
 // (semilag.py:44-59), the reference's operation order.
-template <int C>
-__device__ inline double tricubic(const Grid& g, const TrajVel& V, double xdx, double ydy,
+template <int C, typename TV = TrajVel>
+__device__ inline double tricubic(const Grid& g, const TV& V, double xdx, double ydy,
                                   double zc, int cell) {
     constexpr int P = 4, h = 1;
     double sx = xdx + Comp<C>::shift_x;
@@ -385,7 +399,7 @@ __device__ inline double tricubic(const Grid& g, const TrajVel& V, double xdx, d
         const int ix = wrap_idx(fx + i, g.nx);
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-            const int base = node_addr<C>(g, ix, wrap_idx(fy + j, g.ny), k0);
+            const int base = node_addr<C, TV::packed>(g, ix, wrap_idx(fy + j, g.ny), k0);
             double wij = wx[i] * wy[j];
 #pragma unroll
             for (int k = 0; k < P; ++k) {
@@ -402,8 +416,9 @@ __device__ inline double tricubic(const Grid& g, const TrajVel& V, double xdx, d
 // unclamped.  The three trajectory samples at one point share x/dx, y/dy and
 // the face cell of z (the reference recomputes them with identical results).
 // T: trajectory sampling order (2 trilinear, 4 tricubic).
-template <int T = 2>
-__device__ inline void foot(const Grid& g, const TrajVel& V, double xa, double ya, double za,
+// TV: TrajVel (ghost-shaped fields) or TrajNodes (packed node buffers).
+template <int T = 2, typename TV = TrajVel>
+__device__ inline void foot(const Grid& g, const TV& V, double xa, double ya, double za,
                             double half_dt, double& xd, double& yd, double& zd) {
     double xm = xa, ym = ya, zm = za;
     for (int it = 0; it < g.n_iters; ++it) {
@@ -413,13 +428,13 @@ __device__ inline void foot(const Grid& g, const TrajVel& V, double xa, double y
         int cell = face_cell_fast(g, zc);
         double us, vs, ws;
         if constexpr (T == 4) {
-            us = tricubic<0>(g, V, xdx, ydy, zc, cell);
-            vs = tricubic<1>(g, V, xdx, ydy, zc, cell);
-            ws = tricubic<2>(g, V, xdx, ydy, zc, cell);
+            us = tricubic<0, TV>(g, V, xdx, ydy, zc, cell);
+            vs = tricubic<1, TV>(g, V, xdx, ydy, zc, cell);
+            ws = tricubic<2, TV>(g, V, xdx, ydy, zc, cell);
         } else {
-            us = trilinear<0>(g, V, xdx, ydy, zc, cell);
-            vs = trilinear<1>(g, V, xdx, ydy, zc, cell);
-            ws = trilinear<2>(g, V, xdx, ydy, zc, cell);
+            us = trilinear<0, TV>(g, V, xdx, ydy, zc, cell);
+            vs = trilinear<1, TV>(g, V, xdx, ydy, zc, cell);
+            ws = trilinear<2, TV>(g, V, xdx, ydy, zc, cell);
         }
         xm = xa - half_dt * us;
         ym = ya - half_dt * vs;

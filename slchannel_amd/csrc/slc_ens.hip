@@ -595,6 +595,50 @@ int slc_ens_member_bytes(const slc_grid_desc* d, int storage_bits, int64_t* byte
     return SLC_OK;
 }
 
+// A new member arena of geometry `key` (plan, field sets, tables); on failure
+// nothing stays allocated and the status is returned (SLC_ENOMEM when an
+// allocation failed).
+static int new_member(slc_ens* e, const slc_grid_desc* d, const GeoKey& key, Member** out) {
+    *out = nullptr;
+    Member* m = new Member();
+    m->key = key;
+    int rc = plan_create(d, e->device, false, &m->plan);
+    if (rc) {  // SLC_ENOMEM when a plan buffer did not fit
+        delete m;
+        return rc;
+    }
+    m->nx = d->nx;
+    m->ny = d->ny;
+    m->nz = d->nz;
+    void* p = nullptr;
+    const int nx = m->nx, ny = m->ny, nz = m->nz;
+#define MALLOC(bytes, dst, T)                    \
+    do {                                         \
+        if ((rc = m_alloc(m, (bytes), &p))) {    \
+            free_member(m);                      \
+            return rc;                           \
+        }                                        \
+        dst = static_cast<T*>(p);                \
+    } while (0)
+    for (int s = 0; s < kFieldSets; ++s)
+        for (int c = 0; c < 3; ++c)
+            MALLOC(8 * field_elems(c, nx, ny, nz), m->fld[s][c], double);
+    MALLOC(8 * p_elems(nx, ny, nz), m->p, double);
+    if (e->storage == 32)
+        for (int c = 0; c < 3; ++c) {
+            MALLOC(4 * field_elems(c, nx, ny, nz), m->f32n[c], float);
+            MALLOC(4 * field_elems(c, nx, ny, nz), m->f32m[c], float);
+        }
+    MALLOC(8 * 18 * (size_t)(nz + 2), m->ztab, double);
+    MALLOC(8 * 4, m->scal, double);
+    MALLOC(8 * ES_COUNT, m->stats, double);
+    MALLOC(8, m->ncl, unsigned long long);
+    MALLOC(sizeof(int), m->state, int);
+#undef MALLOC
+    *out = m;
+    return SLC_OK;
+}
+
 int slc_ens_add(slc_ens* e, const slc_grid_desc* d, double dt, double U_bulk, int* id) {
     if (int rc = ens_check(e)) return rc;
     if (!d || !id) return fail(SLC_EINVAL, "null argument");
@@ -618,41 +662,15 @@ int slc_ens_add(slc_ens* e, const slc_grid_desc* d, double dt, double U_bulk, in
             break;
         }
     if (!m) {
-        m = new Member();
-        m->key = key;
-        rc = plan_create(d, e->device, false, &m->plan);
-        if (rc) {
-            delete m;
-            return rc;
+        rc = new_member(e, d, key, &m);
+        if (rc == SLC_ENOMEM && !e->pool.empty()) {
+            // released arenas of other geometries still hold HBM: free them
+            // (hipFree waits for the device) and try once more
+            for (Member* q : e->pool) free_member(q);
+            e->pool.clear();
+            rc = new_member(e, d, key, &m);
         }
-        m->nx = d->nx;
-        m->ny = d->ny;
-        m->nz = d->nz;
-        void* p = nullptr;
-        const int nx = m->nx, ny = m->ny, nz = m->nz;
-#define MALLOC(bytes, dst, T)                    \
-    do {                                         \
-        if ((rc = m_alloc(m, (bytes), &p))) {    \
-            free_member(m);                      \
-            return rc;                           \
-        }                                        \
-        dst = static_cast<T*>(p);                \
-    } while (0)
-        for (int s = 0; s < kFieldSets; ++s)
-            for (int c = 0; c < 3; ++c)
-                MALLOC(8 * field_elems(c, nx, ny, nz), m->fld[s][c], double);
-        MALLOC(8 * p_elems(nx, ny, nz), m->p, double);
-        if (e->storage == 32)
-            for (int c = 0; c < 3; ++c) {
-                MALLOC(4 * field_elems(c, nx, ny, nz), m->f32n[c], float);
-                MALLOC(4 * field_elems(c, nx, ny, nz), m->f32m[c], float);
-            }
-        MALLOC(8 * 18 * (size_t)(nz + 2), m->ztab, double);
-        MALLOC(8 * 4, m->scal, double);
-        MALLOC(8 * ES_COUNT, m->stats, double);
-        MALLOC(8, m->ncl, unsigned long long);
-        MALLOC(sizeof(int), m->state, int);
-#undef MALLOC
+        if (rc) return rc;
     }
     m->plan->g.nu = d->nu;  // the only per-run grid parameter
     m->dt = dt;
@@ -720,6 +738,8 @@ int slc_ens_bytes(const slc_ens* e, int64_t* bytes) {
     if (!bytes) return fail(SLC_EINVAL, "null argument");
     int64_t b = e->class_bytes + (int64_t)e->fft_work_bytes;
     for (const Member* m : e->members) b += m->bytes + m->plan->bytes;
+    // released arenas kept for reuse still hold their HBM
+    for (const Member* m : e->pool) b += m->bytes + m->plan->bytes;
     *bytes = b;
     return SLC_OK;
 }
@@ -871,36 +891,26 @@ int slc_ens_get_option(const slc_ens* e, int option, int* value) {
     }
 }
 
-int slc_ens_autotune(slc_ens* e, int reps, void* stream, char* report, int64_t len) {
-    if (int rc = ens_check(e)) return rc;
-    if (reps < 1) return fail(SLC_EINVAL, "reps must be >= 1");
-    if (e->members.empty()) return fail(SLC_EINVAL, "autotune needs members");
-    DeviceGuard dg(e->device);
-    if (int rc = enter(e, stream)) return rc;
-    hipEvent_t t0 = nullptr, t1 = nullptr;
-    HIP_OK(hipEventCreate(&t0));
-    HIP_OK(hipEventCreate(&t1));
-    std::string rep;
+// The timing sweeps of slc_ens_autotune on e->es; leaves the best tile shape
+// and grid mode in e (rebuilt).  Any failure returns its status with e in
+// whatever state the failing sweep left; the caller restores it.
+static int autotune_sweeps(slc_ens* e, int reps, hipEvent_t t0, hipEvent_t t1,
+                           std::string& rep) {
     int best = e->tile_shape;
     float best_ms = 1e30f;
     int rc = SLC_OK;
     for (int shape = 0; shape < kEnsTileShapes && rc == SLC_OK; ++shape) {
         e->tile_shape = shape;
-        if ((rc = rebuild(e))) break;
+        if ((rc = rebuild(e))) return rc;
         // U* (and the FP32 copies) so the far pass traces real trajectories
-        if (hipError_t err = run_ens_ustar(launch(e, EK_ELEM), pgrid(e, EK_ELEM), e->es)) {
-            rc = fail(SLC_EHIP, "autotune ustar: %s", hipGetErrorString(err));
-            break;
-        }
-        if ((rc = sl_passes(e))) break;  // warm-up
+        if (hipError_t err = run_ens_ustar(launch(e, EK_ELEM), pgrid(e, EK_ELEM), e->es))
+            return fail(SLC_EHIP, "autotune ustar: %s", hipGetErrorString(err));
+        if ((rc = sl_passes(e))) return rc;  // warm-up
         (void)hipEventRecord(t0, e->es);
         for (int r = 0; r < reps && rc == SLC_OK; ++r) rc = sl_passes(e);
         (void)hipEventRecord(t1, e->es);
-        if (rc) break;
-        if (hipEventSynchronize(t1) != hipSuccess) {
-            rc = fail(SLC_EHIP, "autotune timing failed");
-            break;
-        }
+        if (rc) return rc;
+        if (hipEventSynchronize(t1) != hipSuccess) return fail(SLC_EHIP, "autotune timing failed");
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, t0, t1);
         ms /= (float)reps;
@@ -914,14 +924,14 @@ int slc_ens_autotune(slc_ens* e, int reps, void* stream, char* report, int64_t l
     }
     e->tile_shape = best;
     e->dirty = true;
-    if (rc == SLC_OK) rc = rebuild(e);
+    if ((rc = rebuild(e))) return rc;
     // grid of the other batched kernels: time the substeps that only write
     // scratch (ghost fill of V^n, U*, z-solves of the predictor buffer,
     // divergence planes, Thomas)
     int best_grid = e->grid_per_cu;
     float best_gms = 1e30f;
     const int grids[3] = {0, 4, 16};
-    for (int gi = 0; gi < 3 && rc == SLC_OK; ++gi) {
+    for (int gi = 0; gi < 3; ++gi) {
         e->grid_per_cu = grids[gi];
         auto substeps = [&]() -> int {
             hipStream_t s = e->es;
@@ -934,15 +944,12 @@ int slc_ens_autotune(slc_ens* e, int reps, void* stream, char* report, int64_t l
             HIP_OK(run_ens_thomas(launch(e, EK_THOMAS), pgrid(e, EK_THOMAS), s));
             return SLC_OK;
         };
-        if ((rc = substeps())) break;
+        if ((rc = substeps())) return rc;
         (void)hipEventRecord(t0, e->es);
         for (int r = 0; r < reps && rc == SLC_OK; ++r) rc = substeps();
         (void)hipEventRecord(t1, e->es);
-        if (rc) break;
-        if (hipEventSynchronize(t1) != hipSuccess) {
-            rc = fail(SLC_EHIP, "autotune timing failed");
-            break;
-        }
+        if (rc) return rc;
+        if (hipEventSynchronize(t1) != hipSuccess) return fail(SLC_EHIP, "autotune timing failed");
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, t0, t1);
         ms /= (float)reps;
@@ -955,14 +962,36 @@ int slc_ens_autotune(slc_ens* e, int reps, void* stream, char* report, int64_t l
         }
     }
     e->grid_per_cu = best_grid;
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
-    if (rc) return rc;
     // the timed passes accumulated clamp counts: clear them
     HIP_OK(run_ens_clear_ncl(launch(e, EK_ONE), e->es));
-    if (report && len > 0) {
-        snprintf(report, (size_t)len, "%s", rep.c_str());
+    return SLC_OK;
+}
+
+int slc_ens_autotune(slc_ens* e, int reps, void* stream, char* report, int64_t len) {
+    if (int rc = ens_check(e)) return rc;
+    if (reps < 1) return fail(SLC_EINVAL, "reps must be >= 1");
+    if (e->members.empty()) return fail(SLC_EINVAL, "autotune needs members");
+    DeviceGuard dg(e->device);
+    if (int rc = enter(e, stream)) return rc;
+    const int prev_shape = e->tile_shape, prev_grid = e->grid_per_cu;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    std::string rep;
+    int rc = SLC_OK;
+    if (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess)
+        rc = fail(SLC_EHIP, "autotune: hipEventCreate failed");
+    if (rc == SLC_OK) rc = autotune_sweeps(e, reps, t0, t1, rep);
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    if (rc) {
+        // keep the configuration the ensemble had; the launch tables are
+        // rebuilt from it before the next step
+        e->tile_shape = prev_shape;
+        e->grid_per_cu = prev_grid;
+        e->dirty = true;
+        (void)leave(e, stream);
+        return rc;
     }
+    if (report && len > 0) snprintf(report, (size_t)len, "%s", rep.c_str());
     return leave(e, stream);
 }
 

@@ -18,8 +18,11 @@ struct PoissonTabs {
 };
 
 // semi-Lagrangian kernels (slc_sl.hip)
-hipError_t run_departure(const Grid& g, int comp, const TrajVel& V, double dt, double* xd,
-                         double* yd, double* zd, hipStream_t s);
+// layout 0: ghost-shaped trajectory fields; 1: packed node buffers.  ncl
+// non-null: z clamped and the clamped feet counted.
+hipError_t run_departure(const Grid& g, int comp, const TrajVel& V, int layout, double dt,
+                         double* xd, double* yd, double* zd, unsigned long long* ncl,
+                         hipStream_t s);
 hipError_t run_gather(const Grid& g, int comp, const double* F, long fsx, long fsy, int fox,
                       int foy, const double* xd, const double* yd, const double* zd, double* out,
                       unsigned long long* ncl, hipStream_t s);

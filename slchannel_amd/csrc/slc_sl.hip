@@ -35,22 +35,34 @@ __device__ inline void block_count(unsigned long long* counter, int mine) {
     if (threadIdx.x == 0 && s_cnt) atomicAdd(counter, (unsigned long long)s_cnt);
 }
 
-template <int C, int T>
-__global__ __launch_bounds__(kBlock) void departure_kernel(Grid g, TrajVel V, double dt,
+// TV: TrajVel (ghost-shaped trajectory fields) or TrajNodes (packed node
+// buffers).  With nclamp the stored z is clamped into [z_lo, z_hi] and the
+// clamped feet are counted (TritonSL.departure, semilag_triton.py:262-269);
+// without it z is stored unclamped (departure_coords, semilag.py:534-552).
+template <int C, int T, typename TV>
+__global__ __launch_bounds__(kBlock) void departure_kernel(Grid g, TV V, double dt,
                                                            double* __restrict__ xd,
                                                            double* __restrict__ yd,
-                                                           double* __restrict__ zd, long N) {
+                                                           double* __restrict__ zd, long N,
+                                                           unsigned long long* nclamp) {
     long nb = (N + kBlock - 1) / kBlock;
     long n = xcd_block(nb) * kBlock + threadIdx.x;
-    if (n >= N) return;
-    double xa, ya, za;
-    int I, J, K;
-    arrival<C>(g, n, xa, ya, za, I, J, K);
-    double x, y, z;
-    foot<T>(g, V, xa, ya, za, 0.5 * dt, x, y, z);
-    xd[n] = x;
-    yd[n] = y;
-    zd[n] = z;
+    int clamped = 0;
+    if (n < N) {
+        double xa, ya, za;
+        int I, J, K;
+        arrival<C>(g, n, xa, ya, za, I, J, K);
+        double x, y, z;
+        foot<T, TV>(g, V, xa, ya, za, 0.5 * dt, x, y, z);
+        if (nclamp) {
+            clamped = (z < g.z_lo || z > g.z_hi) ? 1 : 0;
+            z = clampd(z, g.z_lo, g.z_hi);
+        }
+        xd[n] = x;
+        yd[n] = y;
+        zd[n] = z;
+    }
+    if (nclamp) block_count(nclamp, clamped);
 }
 
 template <int C, int P>
@@ -287,22 +299,33 @@ static inline dim3 grid_for(long N) { return dim3((unsigned)((N + kBlock - 1) / 
         default: FN<2>(__VA_ARGS__); break; \
     }
 
-template <int C>
-static void launch_departure(const Grid& g, const TrajVel& V, double dt, double* xd, double* yd,
-                             double* zd, hipStream_t s) {
+template <int C, typename TV>
+static void launch_departure_tv(const Grid& g, const TV& V, double dt, double* xd, double* yd,
+                                double* zd, unsigned long long* ncl, hipStream_t s) {
     long N = n_arrival(g, C);
     if (N <= 0) return;
     if (g.traj_order == 4)
-        hipLaunchKernelGGL((departure_kernel<C, 4>), grid_for(N), dim3(kBlock), 0, s, g, V, dt, xd,
-                           yd, zd, N);
+        hipLaunchKernelGGL((departure_kernel<C, 4, TV>), grid_for(N), dim3(kBlock), 0, s, g, V,
+                           dt, xd, yd, zd, N, ncl);
     else
-        hipLaunchKernelGGL((departure_kernel<C, 2>), grid_for(N), dim3(kBlock), 0, s, g, V, dt, xd,
-                           yd, zd, N);
+        hipLaunchKernelGGL((departure_kernel<C, 2, TV>), grid_for(N), dim3(kBlock), 0, s, g, V,
+                           dt, xd, yd, zd, N, ncl);
 }
 
-hipError_t run_departure(const Grid& g, int comp, const TrajVel& V, double dt, double* xd,
-                         double* yd, double* zd, hipStream_t s) {
-    SLC_DISPATCH_C(comp, launch_departure, g, V, dt, xd, yd, zd, s);
+template <int C>
+static void launch_departure(const Grid& g, const TrajVel& V, int layout, double dt, double* xd,
+                             double* yd, double* zd, unsigned long long* ncl, hipStream_t s) {
+    if (layout == 1)
+        launch_departure_tv<C, TrajNodes>(g, TrajNodes{{V.a[0], V.a[1], V.a[2]}}, dt, xd, yd, zd,
+                                          ncl, s);
+    else
+        launch_departure_tv<C, TrajVel>(g, V, dt, xd, yd, zd, ncl, s);
+}
+
+hipError_t run_departure(const Grid& g, int comp, const TrajVel& V, int layout, double dt,
+                         double* xd, double* yd, double* zd, unsigned long long* ncl,
+                         hipStream_t s) {
+    SLC_DISPATCH_C(comp, launch_departure, g, V, layout, dt, xd, yd, zd, ncl, s);
     return hipGetLastError();
 }
 

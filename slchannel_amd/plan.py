@@ -164,21 +164,34 @@ class HipPlan:
         _lib.check(self._lib.slc_apply_bc(self._h, _ptr(u), _ptr(v), _ptr(w), self._stream()),
                    "slc_apply_bc")
 
-    def departure(self, comp, um, vm, wm, dt, xd, yd, zd):
-        self._uvw(um, vm, wm, "_mid")
+    def node_shape(self, comp):
+        """Packed node buffer of a component, (nx, ny, NZ) (semilag.py:363-372)."""
+        return (self.nx, self.ny, self.nz + 1 if comp == "w" else self.nz + 2)
+
+    def departure(self, comp, um, vm, wm, dt, xd, yd, zd, nclamp=None, layout=0):
+        """Feet of `comp` (include/slc.h slc_departure).  layout 0: ghost-shaped
+        trajectory fields; 1: packed node buffers.  With nclamp, z is clamped
+        and the clamped feet are counted."""
+        if layout == 0:
+            self._uvw(um, vm, wm, "_mid")
+        elif layout == 1:
+            for t, c in ((um, "u"), (vm, "v"), (wm, "w")):
+                self._chk(t, self.node_shape(c), f"mbuf[{c}]")
+        else:
+            raise ValueError(f"layout must be 0 or 1, got {layout}")
         n = self.n_arrival(comp)
         for t, nm in ((xd, "xd"), (yd, "yd"), (zd, "zd")):
             self._chk(t, (n,), nm)
         _lib.check(self._lib.slc_departure(self._h, COMPS.index(comp), _ptr(um), _ptr(vm),
-                                           _ptr(wm), float(dt), _ptr(xd), _ptr(yd), _ptr(zd),
-                                           self._stream()), "slc_departure")
+                                           _ptr(wm), int(layout), float(dt), _ptr(xd), _ptr(yd),
+                                           _ptr(zd), self._counter(nclamp), self._stream()),
+                   "slc_departure")
 
     def gather(self, comp, field, xd, yd, zd, out, nclamp=None, layout=0):
         if layout == 0:
             self._chk(field, self.shapes[comp], "field")
         else:
-            NZ = self.nz + 1 if comp == "w" else self.nz + 2
-            self._chk(field, (self.nx, self.ny, NZ), "node buffer")
+            self._chk(field, self.node_shape(comp), "node buffer")
         n = self.n_arrival(comp)
         for t, nm in ((xd, "xd"), (yd, "yd"), (zd, "zd"), (out, "out")):
             self._chk(t, (n,), nm)

@@ -25,11 +25,35 @@ from .plan import HipPlan
 
 
 class HipSL:
-    """Launch wrapper bound to one SLAdvector (TritonSL's method shape)."""
+    """Launch wrapper bound to one advector (TritonSL's method shape,
+    slchannel/semilag_triton.py:347-455).
+
+    ``adv`` is either this package's ``SLAdvector`` (its plan is reused) or
+    the reference's ``slchannel.semilag.SLAdvector`` (a plan is built from its
+    grid attributes, as ``TritonSL.__init__`` builds its tables).  The call
+    protocol is the reference's (semilag.py:617-666): ``nclamp.zero_()``, then
+    per component ``n = departure(comp, dt)`` -- feet of the trajectory
+    velocity, z clamped, clamped feet counted into ``nclamp`` -- and
+    ``gather(comp, buf, n)`` on a node buffer or a ghost-shaped field, whose
+    returned tensor is reused by the next gather of the component.
+
+    The trajectory velocity ``departure`` samples is, in order: the
+    ghost-shaped triple of the last ``set_mid`` call, else the advector's
+    node buffers ``adv.mbuf`` (semilag.py:604-606, filled by the reference's
+    ``advect`` before it calls ``departure``).  Everything is float64.
+    """
 
     def __init__(self, adv):
         self.adv = adv
-        plan = adv.plan
+        plan = getattr(adv, "plan", None)
+        if plan is None:
+            plan = HipPlan(adv.nx, adv.ny, adv.nz, adv.Lx, adv.Ly, adv.Lz, adv.z_f, adv.z_c,
+                           adv.gamma, stretching_type=adv.stretching_type, order=adv.order,
+                           traj_order=getattr(adv, "traj_order", 2),
+                           n_traj_iters=adv.n_traj_iters,
+                           top_wall_bc_type=getattr(adv, "top_wall_bc_type", "dirichlet"),
+                           tanh_g=float(adv._tanh_g), device=adv.device)
+        self.plan = plan
         dev = plan.device
         n_max = plan.nx * plan.ny * plan.nz
         self.xd = torch.empty(n_max, dtype=torch.float64, device=dev)
@@ -41,27 +65,41 @@ class HipSL:
         self._mid = None
 
     def set_mid(self, u_mid, v_mid, w_mid):
-        """Trajectory velocity (ghost-shaped, BC-consistent) used by departure()."""
-        self._mid = (u_mid, v_mid, w_mid)
+        """Ghost-shaped, BC-consistent trajectory velocity for departure();
+        ``set_mid(None, None, None)`` returns to the advector's node buffers."""
+        self._mid = None if u_mid is None else (u_mid, v_mid, w_mid)
+
+    def _trajectory(self):
+        if self._mid is not None:
+            return self._mid, 0
+        mbuf = getattr(self.adv, "mbuf", None)
+        if mbuf is None:
+            raise RuntimeError("HipSL.departure: no trajectory velocity (call set_mid, or bind "
+                               "an advector with node buffers mbuf)")
+        return (mbuf["u"], mbuf["v"], mbuf["w"]), 1
 
     def departure(self, comp, dt):
-        """Feet of component `comp` for depth dt into self.xd/yd/zd; returns n."""
-        if self._mid is None:
-            raise RuntimeError("HipSL.departure: no trajectory velocity set (call set_mid)")
-        plan = self.adv.plan
+        """Feet of component `comp` for depth dt into self.xd/yd/zd, z clamped
+        into [z_lo, z_hi]; the clamped feet are added to self.nclamp.  Returns
+        the number of arrival points n."""
+        plan = self.plan
         n = plan.n_arrival(comp)
-        plan.departure(comp, *self._mid, float(dt), self.xd[:n], self.yd[:n], self.zd[:n])
+        (um, vm, wm), layout = self._trajectory()
+        plan.departure(comp, um, vm, wm, float(dt), self.xd[:n], self.yd[:n], self.zd[:n],
+                       nclamp=self.nclamp, layout=layout)
         return n
 
     def gather(self, comp, buf, n):
         """Interpolate `buf` (ghost-shaped field or (nx, ny, NZ) node buffer) at
-        the last feet; counts clamped feet into self.nclamp.  Returns the
-        reused (arrival-shaped) output buffer of `comp`."""
-        plan = self.adv.plan
+        the last feet (clamps were counted by departure()).  Returns the reused
+        arrival-shaped output buffer of `comp`."""
+        plan = self.plan
+        if n != plan.n_arrival(comp):
+            raise ValueError(f"HipSL.gather: n={n}, component {comp} has "
+                             f"{plan.n_arrival(comp)} arrival points")
         layout = 0 if tuple(buf.shape) == plan.shapes[comp] else 1
         out = self.out[comp].reshape(-1)
-        plan.gather(comp, buf, self.xd[:n], self.yd[:n], self.zd[:n], out, self.nclamp,
-                    layout=layout)
+        plan.gather(comp, buf, self.xd[:n], self.yd[:n], self.zd[:n], out, None, layout=layout)
         return self.out[comp]
 
 
@@ -119,11 +157,13 @@ class SLAdvector:
 
     def departure_coords(self, comp, dt_t, u_mid, v_mid, w_mid):
         """Feet (x, y unwrapped; z not clamped) of `comp`, each (nx, ny, nzi)."""
-        self._hip.set_mid(u_mid, v_mid, w_mid)
-        n = self._hip.departure(comp, float(dt_t))
-        shape = self.plan.arrival_shape(comp)
-        return tuple(t[:n].clone().reshape(shape) for t in (self._hip.xd, self._hip.yd,
-                                                             self._hip.zd))
+        plan = self.plan
+        n = plan.n_arrival(comp)
+        hip = self._hip
+        plan.departure(comp, u_mid, v_mid, w_mid, float(dt_t), hip.xd[:n], hip.yd[:n],
+                       hip.zd[:n])
+        shape = plan.arrival_shape(comp)
+        return tuple(t[:n].clone().reshape(shape) for t in (hip.xd, hip.yd, hip.zd))
 
     def advect(self, u, v, w, u_mid, v_mid, w_mid, dt_t, extra_rhs=None):
         """semilag.py:584-671.  Returns (ustar, vstar, wstar) -- persistent
@@ -145,18 +185,19 @@ class SLAdvector:
         extras = [dict() for _ in extra_rhs]
         nx, ny, nz = self.nx, self.ny, self.nz
         for ic, comp in enumerate("uvw"):
+            # departure() counts the clamped feet once per component; the
+            # extras are gathered at the same feet first and cloned, since
+            # gather() reuses one output buffer (semilag.py:622-638)
             n = hip.departure(comp, dt)
             shape = plan.arrival_shape(comp)
             for t, triple in enumerate(extra_rhs):
-                # gather() does not count clamps for the extras (same feet)
-                out = torch.empty(n, dtype=torch.float64, device=self.device)
-                plan.gather(comp, triple[ic], hip.xd[:n], hip.yd[:n], hip.zd[:n], out, None)
-                extras[t][comp] = out.reshape(shape)
+                extras[t][comp] = hip.gather(comp, triple[ic], n).clone().reshape(shape)
             vals = hip.gather(comp, fields[comp], n)
             if comp == "w":
                 outs[comp][1:nx + 1, 1:ny + 1, 1:nz] = vals
             else:
                 outs[comp][1:nx + 1, 1:ny + 1, 1:nz + 1] = vals
+        hip.set_mid(None, None, None)
         self.n_clamped_last = hip.nclamp[0].clone()
         return (self.ustar, self.vstar, self.wstar,
                 [(e["u"], e["v"], e["w"]) for e in extras])

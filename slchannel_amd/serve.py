@@ -67,7 +67,9 @@ class Result:
     fields: dict = None
     stats: list = None
     diag: list = None         # [max|div|, u_tau, CFL, dti, non-finite] at completion
-    status: str = "pending"   # ok | rejected_nonfinite | stopped_nonfinite | stopped_blowup
+    # ok | rejected_invalid (diag = [reason]) | rejected_nonfinite |
+    # stopped_nonfinite | stopped_blowup
+    status: str = "pending"
 
 
 def _grid_key(config, device=None):
@@ -86,10 +88,12 @@ def _prepare_async(req, device):
     via slc_diagnostics) for the caller to read once the stream is done."""
     probe = torch.zeros(2, dtype=torch.float64, device=device)
     if req.fields is not None:
-        u, v, w = (t.to(device=device, dtype=torch.float64).contiguous() for t in req.fields[:3])
-        p = req.fields[3] if len(req.fields) > 3 else None
         if req.dt is None:
             raise ValueError(f"request {req.rid}: explicit fields need an explicit dt")
+        u, v, w = (t.to(device=device, dtype=torch.float64).contiguous() for t in req.fields[:3])
+        p = req.fields[3] if len(req.fields) > 3 and req.fields[3] is not None else None
+        if p is not None:
+            p = p.to(device=device, dtype=torch.float64).contiguous()
         bad = sum((~torch.isfinite(t)).sum() for t in (u, v, w, p) if t is not None)
         probe[1:2].copy_(bad.to(torch.float64).reshape(1))
         return (u, v, w, p), float(req.dt), probe
@@ -101,6 +105,62 @@ def _prepare_async(req, device):
     _INIT_CACHE[_grid_key(req.config, device)].plan.diagnostics(u, v, w, None, 0.0, out)
     probe[0:1].copy_(out[3:4])
     return (u, v, w, p), None, probe
+
+
+def validate(req):
+    """Why a request cannot be served (None if it can): a config the ensemble
+    does not run, a non-positive step count or dt, or given fields without a
+    dt or of the wrong shapes.  Checked before the loop starts, so one bad
+    request is rejected on its own instead of aborting the run."""
+    try:
+        c = _config_grid(req.config)
+    except (KeyError, TypeError, ValueError) as e:
+        return f"config: {e}"
+    if int(req.n_steps) < 1:
+        return f"n_steps must be >= 1, got {req.n_steps}"
+    if req.dt is not None and not (float(req.dt) > 0.0):
+        return f"dt must be positive, got {req.dt}"
+    if req.fields is not None:
+        if req.dt is None:
+            return "explicit fields need an explicit dt"
+        from .plan import field_shapes
+
+        shapes = field_shapes(c["nx"], c["ny"], c["nz"])
+        if len(req.fields) not in (3, 4):
+            return "fields must be (u, v, w) or (u, v, w, p)"
+        for name, t in zip("uvwp", req.fields):
+            if t is None and name == "p":
+                continue
+            if not isinstance(t, torch.Tensor) or not t.is_floating_point():
+                return f"field {name}: a floating-point tensor is required"
+            if tuple(t.shape) != shapes[name]:
+                return f"field {name}: shape {tuple(t.shape)}, expected {shapes[name]}"
+    return None
+
+
+def init_cache_bytes(device):
+    """Device bytes held by the per-grid initialisers of `device` (their plans
+    and fields), which share the HBM budget with the members."""
+    def tensors(obj):
+        out = []
+        for t in vars(obj).values():
+            if isinstance(t, torch.Tensor) and t.is_cuda:
+                out.append(t)
+            elif isinstance(t, (list, tuple)):
+                out.extend(x for x in t if isinstance(x, torch.Tensor) and x.is_cuda)
+        return out
+
+    total = 0
+    for key, flow in _INIT_CACHE.items():
+        if key[-1] != str(device):
+            continue
+        total += flow.plan.workspace_bytes()
+        seen = {}
+        for obj in (flow, flow.sl, flow.sl._hip):
+            for t in tensors(obj):
+                seen[t.data_ptr()] = t.numel() * t.element_size()
+        total += sum(seen.values())
+    return total
 
 
 def _dt_of(req, dti):
@@ -160,9 +220,14 @@ class Server:
                 initial_fields(cfg, 0, device=self.device)
         torch.cuda.synchronize(self.device)
 
+    def footprint(self):
+        """HBM the server holds: the ensemble (live members, pooled arenas,
+        class buffers) plus the per-grid initialisers."""
+        return self.ens.bytes() + init_cache_bytes(self.device)
+
     def fits(self, req, reserved=0, preparing=0):
         need = member_bytes(req.config, self.storage)
-        return (need <= self.budget - self.ens.bytes() - reserved
+        return (need <= self.budget - self.footprint() - reserved
                 and len(self.ens) + preparing < self.max_members)
 
     def run(self, requests, log=None, inflight=3):
@@ -171,11 +236,18 @@ class Server:
         at most `inflight` batched steps queued ahead of the device and only
         synchronises to retire a finished run; each step's latency is its
         device time from hipEvents."""
-        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
-        for r in pending:
-            if member_bytes(r.config, self.storage) > self.budget:
-                raise ValueError(f"request {r.rid} alone exceeds the HBM budget")
         results = {}
+        pending = []
+        for r in sorted(requests, key=lambda r: (r.arrival, r.rid)):
+            why = validate(r)
+            if why is None and member_bytes(r.config, self.storage) > self.budget:
+                why = "alone exceeds the HBM budget"
+            if why is None:
+                pending.append(r)
+            else:
+                results[r.rid] = Result(rid=r.rid, arrival=r.arrival, admitted=r.arrival,
+                                        finished=r.arrival, status="rejected_invalid")
+                results[r.rid].diag = [why]
         active = {}  # member id -> (request, result)
         queued = []  # (start event, end event, results stepped)
         retiring = []  # (event after the copies out, request, result, pinned stats, diag)
@@ -238,8 +310,19 @@ class Server:
                 if log:
                     c = _config_grid(req.config)
                     log(f"admit {req.rid} {c['nx']}x{c['ny']}x{c['nz']} "
-                        f"steps={req.n_steps} active={len(active)}")
+                        f"steps={req.n_steps} footprint={self.footprint()} "
+                        f"active={len(active)}")
             host["admit_s"] += time.perf_counter() - ta
+            if not active and not preparing and pending and pending[0].arrival <= now and \
+                    not self.fits(pending[0]):
+                # nothing runs that could free memory: the server's fixed
+                # footprint (pooled arenas, initialisers) leaves no room
+                req = pending.pop(0)
+                res = Result(rid=req.rid, arrival=req.arrival, status="rejected_invalid")
+                res.admitted = res.finished = time.perf_counter() - t0
+                res.diag = ["does not fit the HBM budget beside the server's fixed footprint"]
+                results[req.rid] = res
+                continue
             if not active:
                 if preparing:
                     time.sleep(1e-4)

@@ -59,9 +59,19 @@ def test_null_and_invalid_arguments_return_einval(check):
     rc = lib.slc_plan_create(ctypes.byref(desc), 0, ctypes.byref(h))
     check("order 5 rejected", rc == -1 and b"order" in lib.slc_last_error(), f"rc={rc}")
     desc.order = 6
+    desc.traj_order = 3
+    rc = lib.slc_plan_create(ctypes.byref(desc), 0, ctypes.byref(h))
+    check("traj_order 3 rejected", rc == -1 and b"traj_interp_order" in lib.slc_last_error(),
+          f"rc={rc} {lib.slc_last_error()!r}")
+    # traj_order 4 (tricubic) is a supported option: what stops this
+    # descriptor is the missing z tables, checked before any device work
     desc.traj_order = 4
     rc = lib.slc_plan_create(ctypes.byref(desc), 0, ctypes.byref(h))
-    check("traj_order 4 rejected", rc == -1, f"rc={rc} {lib.slc_last_error()!r}")
+    check("traj_order 4 accepted; missing z tables rejected",
+          rc == -1 and b"z_f / z_c host tables required" in lib.slc_last_error(),
+          f"rc={rc} {lib.slc_last_error()!r}")
+    rc = lib.slc_departure(None, 0, None, None, None, 0, 0.1, None, None, None, None, None)
+    check("departure: null plan rejected", rc == -1, f"rc={rc}")
     rc = lib.slc_apply_bc(None, None, None, None, None)
     check("null plan rejected", rc == -1, f"rc={rc}")
 

@@ -35,20 +35,27 @@ def test_results_same_alone_and_in_the_mix(check):
             _req(2, 24, 16, 24, 3500.0, 6, 0.01), _req(3, 16, 16, 32, 3000.0, 5, 0.02),
             _req(4, 32, 24, 16, 5000.0, 3, 0.03)]
     sizes = [member_bytes(r.config) for r in reqs]
-    # room for two or three at once: later requests wait for a finished run
-    budget = int(1.5 * (sorted(sizes)[-1] + sorted(sizes)[-2]))
-    srv = Server(hbm_budget=budget, keep_fields=True)
+    srv = Server(hbm_budget=1 << 40, keep_fields=True)
+    srv.warmup([r.config for r in reqs])
+    # room for two or three at once beside the server's fixed footprint (the
+    # per-grid initialisers): later requests wait for a finished run
+    fixed = srv.footprint()
+    srv.budget = fixed + int(1.5 * (sorted(sizes)[-1] + sorted(sizes)[-2]))
     active_max = [0]
+    foot_max = [0]
     log = []
 
     def note(msg):
         log.append(msg)
         active_max[0] = max(active_max[0], int(msg.split("active=")[1]))
+        foot_max[0] = max(foot_max[0], int(msg.split("footprint=")[1].split()[0]))
 
     results, metrics = srv.run(reqs, log=note)
     check("every request served", sorted(results) == [0, 1, 2, 3, 4], f"{sorted(results)}")
     check("admission bounded by the HBM budget", 1 < active_max[0] < len(reqs),
           f"max concurrent={active_max[0]} log={log}")
+    check("footprint incl. pooled arenas and initialisers <= budget",
+          fixed < foot_max[0] <= srv.budget, f"max {foot_max[0]} budget {srv.budget}")
     joined_mid = any(r.admitted > 0 and r.admitted < max(x.finished for x in results.values())
                      for r in results.values())
     check("runs joined while others were mid-flight", joined_mid, "")
@@ -97,3 +104,26 @@ def test_guards_stop_bad_runs_and_spare_the_rest(check):
           f"{r0.status} diag={r0.diag}")
     check("final diagnostics finite, divergence-free",
           r0.diag[4] == 0.0 and r0.diag[0] < 1e-11 and r0.diag[1] > 0, f"{r0.diag}")
+
+
+def test_invalid_requests_rejected_alone(check):
+    """A malformed request is rejected into its Result; the others run."""
+    from slchannel_amd.serve import Server
+
+    good = _req(0, 24, 16, 24, 2800.0, 3)
+    no_dt = _req(1, 24, 16, 24, 2800.0, 3)
+    no_dt.fields = tuple(torch.zeros(s, dtype=torch.float64) for s in
+                         ((25, 18, 26), (26, 17, 26), (26, 18, 25)))
+    bad_shape = _req(2, 24, 16, 24, 2800.0, 3)
+    bad_shape.fields = (torch.zeros(3, 3, 3),) * 3
+    bad_shape.dt = 0.01
+    tricubic = _req(3, 24, 16, 24, 2800.0, 3)
+    tricubic.config["sl"]["traj_interp_order"] = 4
+    srv = Server()
+    results, _ = srv.run([good, no_dt, bad_shape, tricubic])
+    check("good request ok", results[0].status == "ok" and results[0].steps == 3,
+          results[0].status)
+    for rid in (1, 2, 3):
+        r = results[rid]
+        check(f"request {rid} rejected_invalid", r.status == "rejected_invalid" and r.steps == 0,
+              f"{r.status} {r.diag}")
