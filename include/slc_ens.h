@@ -52,8 +52,13 @@ int slc_ens_member_bytes(const slc_grid_desc* desc, int storage_bits, int64_t* b
 int slc_ens_add(slc_ens* ens, const slc_grid_desc* desc, double dt, double U_bulk, int* id);
 int slc_ens_remove(slc_ens* ens, int id);
 int slc_ens_count(const slc_ens* ens, int* n);
-/* Total device bytes held by the ensemble. */
+/* Total device bytes held by the ensemble (members, class buffers, FFT work
+ * area, and the released member arenas kept for reuse). */
 int slc_ens_bytes(const slc_ens* ens, int64_t* bytes);
+/* Frees released member arenas, oldest first, until the pool holds at most
+ * keep_bytes; *pool_bytes (optional) receives what the pool still holds.
+ * keep_bytes < 0 only reports.  Freeing waits for the device (hipFree). */
+int slc_ens_trim_pool(slc_ens* ens, int64_t keep_bytes, int64_t* pool_bytes);
 
 /* Copies ghost-shaped u, v, w, p (device pointers, float64, SLChannelFlow
  * shapes) into the member and resets it to a BDF1 bootstrap (a restart,

@@ -24,7 +24,8 @@ EXPORTS = (
     "slc_ab2_update", "slc_set_option",
     "slc_profile_enable", "slc_profile_report",
     "slc_ens_create", "slc_ens_destroy", "slc_ens_member_bytes", "slc_ens_add",
-    "slc_ens_remove", "slc_ens_count", "slc_ens_bytes", "slc_ens_set_fields",
+    "slc_ens_remove", "slc_ens_count", "slc_ens_bytes", "slc_ens_trim_pool",
+    "slc_ens_set_fields",
     "slc_ens_get_fields", "slc_ens_stats", "slc_ens_step", "slc_ens_set_option",
     "slc_ens_get_option", "slc_ens_autotune", "slc_ens_diagnostics",
     "slc_stats_create", "slc_stats_destroy", "slc_stats_workspace_bytes",
@@ -97,6 +98,7 @@ def load():
         "slc_ens_remove": ([vp, i], i),
         "slc_ens_count": ([vp, ctypes.POINTER(i)], i),
         "slc_ens_bytes": ([vp, ctypes.POINTER(ctypes.c_int64)], i),
+        "slc_ens_trim_pool": ([vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)], i),
         "slc_ens_set_fields": ([vp, i, vp, vp, vp, vp, vp], i),
         "slc_ens_get_fields": ([vp, i, vp, vp, vp, vp, vp], i),
         "slc_ens_stats": ([vp, i, vp, vp], i),

@@ -744,6 +744,23 @@ int slc_ens_bytes(const slc_ens* e, int64_t* bytes) {
     return SLC_OK;
 }
 
+int slc_ens_trim_pool(slc_ens* e, int64_t keep_bytes, int64_t* pool_bytes) {
+    if (int rc = ens_check(e)) return rc;
+    DeviceGuard dg(e->device);
+    auto held = [e]() {
+        int64_t b = 0;
+        for (const Member* m : e->pool) b += m->bytes + m->plan->bytes;
+        return b;
+    };
+    if (keep_bytes >= 0)
+        while (!e->pool.empty() && held() > keep_bytes) {
+            free_member(e->pool.front());
+            e->pool.erase(e->pool.begin());
+        }
+    if (pool_bytes) *pool_bytes = held();
+    return SLC_OK;
+}
+
 int slc_ens_set_fields(slc_ens* e, int id, const double* u, const double* v, const double* w,
                        const double* p, void* stream) {
     if (int rc = ens_check(e)) return rc;

@@ -145,6 +145,15 @@ class Ensemble:
         _lib.check(self._lib.slc_ens_bytes(self._h, ctypes.byref(out)), "slc_ens_bytes")
         return out.value
 
+    def trim_pool(self, keep_bytes=-1):
+        """Free released member arenas (oldest first) until the pool holds at
+        most keep_bytes (-1: free nothing); returns the bytes it still holds."""
+        out = ctypes.c_int64()
+        with torch.cuda.device(self.device):
+            _lib.check(self._lib.slc_ens_trim_pool(self._h, int(keep_bytes), ctypes.byref(out)),
+                       "slc_ens_trim_pool")
+        return out.value
+
     # -- fields ----------------------------------------------------------------
 
     def _checked(self, mid, name, t):

@@ -225,10 +225,22 @@ class Server:
         class buffers) plus the per-grid initialisers."""
         return self.ens.bytes() + init_cache_bytes(self.device)
 
-    def fits(self, req, reserved=0, preparing=0):
+    def fits(self, req, reserved=0, preparing=0, trim=False):
+        """Whether the request fits the budget beside what the server holds.
+        Released arenas kept for reuse count as held; with trim=True, the
+        oldest of them are freed when that makes the request fit."""
+        if len(self.ens) + preparing >= self.max_members:
+            return False
         need = member_bytes(req.config, self.storage)
-        return (need <= self.budget - self.footprint() - reserved
-                and len(self.ens) + preparing < self.max_members)
+        room = self.budget - self.footprint() - reserved
+        if need <= room:
+            return True
+        pooled = self.ens.trim_pool(-1)
+        if need > room + pooled:
+            return False
+        if trim:
+            self.ens.trim_pool(pooled - (need - room))
+        return True
 
     def run(self, requests, log=None, inflight=3):
         """Serve `requests` (arrival times relative to the start); returns
@@ -278,7 +290,7 @@ class Server:
             # finiteness check) through pinned memory once that stream is done
             reserved = sum(x[6] for x in preparing)
             if pending and pending[0].arrival <= now and \
-                    self.fits(pending[0], reserved, len(preparing)):
+                    self.fits(pending[0], reserved, len(preparing), trim=True):
                 req = pending.pop(0)
                 st = self._stream_for(req.config)
                 with torch.cuda.stream(st):

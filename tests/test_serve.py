@@ -51,7 +51,9 @@ def test_results_same_alone_and_in_the_mix(check):
         foot_max[0] = max(foot_max[0], int(msg.split("footprint=")[1].split()[0]))
 
     results, metrics = srv.run(reqs, log=note)
-    check("every request served", sorted(results) == [0, 1, 2, 3, 4], f"{sorted(results)}")
+    check("every request served", sorted(results) == [0, 1, 2, 3, 4] and
+          all(r.status == "ok" for r in results.values()),
+          f"{[(k, r.status, r.diag) for k, r in sorted(results.items())]}")
     check("admission bounded by the HBM budget", 1 < active_max[0] < len(reqs),
           f"max concurrent={active_max[0]} log={log}")
     check("footprint incl. pooled arenas and initialisers <= budget",
