@@ -24,7 +24,16 @@ functions whose results no closed form or invariant fixes bit-for-bit:
                      cases with the non-default BDF2 options (bdf2_pressure
                      'inc', bdf2_xy_rhs 'lagged', solver.py:256-281);
 * ``*_traj4.npz``    advect and step with tricubic trajectory sampling
-                     (traj_interp_order 4, semilag.py:179-209, 512-528).
+                     (traj_interp_order 4, semilag.py:179-209, 512-528);
+* ``imex_small.npz`` compute_momentum_rhs_fused_imex (operators.py:639-835) on
+                     both stretchings, and three SLChannelFlow.step_imex steps
+                     (solver.py:1055-1134) with that right-hand side (the
+                     path the reference takes on a GPU, solver.py:1027-1039);
+* ``stats_small.npz`` TurbulenceStats.accumulate_statistics over three
+                     snapshots, with two extra spectra planes, and
+                     finalize_statistics (turbstats.py:35-467);
+* ``diag_small.npz`` compute_u_tau (utils.py:116-146, both top-wall types)
+                     and compute_cfl_fused (operators.py:839-940).
 
 Inputs are generated here with numpy (seeded) and stored next to the outputs,
 so the fixtures are self-contained: data only, no reference code.
@@ -287,6 +296,168 @@ def gen_step(ref, name, order, stretching="symmetric", top="dirichlet", n_steps=
     np.savez_compressed(os.path.join(HERE, f"step_{name}.npz"), **d)
 
 
+def _stepper_config(tmp, nx, ny, nz, Lx, Ly, Lz, gamma, Re, dt, stretching, top, advection,
+                    sl=None):
+    import yaml
+
+    cfg = {
+        "grid": {"nx": nx, "ny": ny, "nz": nz},
+        "domain": {"Lx": Lx, "Ly": Ly, "Lz": Lz, "stretching_type": stretching},
+        "flow": {"Re": Re, "Re_tau": 180.0, "U_bulk": 1.0, "gamma": gamma},
+        "boundary_conditions": {"top_wall": {"type": top}},
+        "initialization": {"type": "parabolic", "perturbation_intensity": 0.0},
+        "solver": {"type": "fft"},
+        "time": {"dt": dt, "n_steps": 10, "t_max": 1e9, "CFL_target": 3.0,
+                 "dt_update_interval": 0, "dt_max": 10.0, "dt_min": 1e-6},
+        "compute": {"device": "cpu"},
+        "output": {"results_folder": os.path.join(tmp, "results"), "n_out": 10**8,
+                   "n_save": 10**8},
+        "advection": {"scheme": advection},
+        "statistics": {"n_stats": 0},
+    }
+    if sl is not None:
+        cfg["sl"] = sl
+    path = os.path.join(tmp, "c.yaml")
+    with open(path, "w") as f:
+        yaml.safe_dump(cfg, f)
+    return path
+
+
+def gen_imex(ref):
+    torch, semilag, solver_mod, utils, projection, operators = ref
+    nx, ny, nz = 12, 10, 16
+    Lx, Ly, Lz, gamma = 4 * math.pi, 2 * math.pi, 2.0, 1.5
+    dx, dy = Lx / nx, Ly / ny
+    d = {"nx": nx, "ny": ny, "nz": nz, "Lx": Lx, "Ly": Ly, "Lz": Lz, "gamma": gamma}
+    rng = np.random.default_rng(21)
+    nu = 1.0 / 400.0
+    for stretching, top in (("symmetric", "dirichlet"), ("bottom", "neumann")):
+        tag = "sym" if stretching == "symmetric" else "bot"
+        z_f, z_c, dz_f, dz_c = utils.generate_grid(gamma, nz, Lz, stretching_type=stretching)
+        zf, zc = z_f.numpy(), z_c.numpy()
+        base = {"u": lambda X, Y, Z: 1.5 * (1.0 - (Z - 1.0) ** 2)}
+        u, v, w = make_fields(rng, nx, ny, nz, dx, dy, Lx, Ly, Lz, zf, zc, base=base,
+                              amp=0.3, wamp=0.2)
+        tu, tv, tw = (torch.from_numpy(a.copy()) for a in (u, v, w))
+        solver_mod.apply_bc_all(tu, tv, tw, top)
+        ru, rv, rw = operators.compute_momentum_rhs_fused_imex(
+            tu, tv, tw, nx, ny, nz, dx, dy, dz_c, dz_f, nu)
+        d[f"{tag}_top"] = top
+        d[f"{tag}_nu"] = nu
+        d[f"{tag}_z_f"], d[f"{tag}_z_c"] = zf, zc
+        d[f"{tag}_u"], d[f"{tag}_v"], d[f"{tag}_w"] = (t.numpy().copy() for t in (tu, tv, tw))
+        d[f"{tag}_ru"], d[f"{tag}_rv"], d[f"{tag}_rw"] = (t.numpy().copy() for t in (ru, rv, rw))
+
+    # three IMEX steps from a seeded projected state
+    Re, dt, n_steps = 400.0, 0.05, 3
+    with tempfile.TemporaryDirectory() as tmp:
+        path = _stepper_config(tmp, nx, ny, nz, Lx, Ly, Lz, gamma, Re, dt, "symmetric",
+                               "dirichlet", "eulerian")
+        s = solver_mod.SLChannelFlow(config_file=path)
+        # the right-hand side the reference computes on a GPU (solver.py:1027-1039)
+        s.compute_momentum_rhs_explicit_imex = lambda: operators.compute_momentum_rhs_fused_imex(
+            s.u, s.v, s.w, s.nx, s.ny, s.nz, s.dx, s.dy, s.dz_c, s.dz_f, s.nu)
+        zf, zc = s.z_f.numpy(), s.z_c.numpy()
+        base = {"u": lambda X, Y, Z: 1.5 * (1.0 - (Z - 1.0) ** 2)}
+        u, v, w = make_fields(rng, nx, ny, nz, s.dx, s.dy, Lx, Ly, Lz, zf, zc, base=base,
+                              amp=0.25, wamp=0.15)
+        tu, tv, tw = (torch.from_numpy(a.copy()) for a in (u, v, w))
+        solver_mod.apply_bc_all(tu, tv, tw, "dirichlet")
+        div = utils.compute_divergence(tu, tv, tw, nx, ny, nz, s.dx, s.dy, s.dz_f)
+        p = projection.solve_poisson_fft(div, s.fft_data).clone()
+        tu, tv, tw = projection.project_velocity(tu, tv, tw, p, nx, ny, nz, s.dx, s.dy,
+                                                 s.dz_c, s.dz_f, 1.0)
+        solver_mod.apply_bc_all(tu, tv, tw, "dirichlet")
+        s.u, s.v, s.w, s.p = tu, tv, tw, p
+        s.forcing = 0.0
+        d.update({"step_Re": Re, "step_dt": dt, "step_n": n_steps, "step_z_f": zf,
+                  "step_z_c": zc, "step_u0": tu.numpy().copy(), "step_v0": tv.numpy().copy(),
+                  "step_w0": tw.numpy().copy(), "step_p0": p.numpy().copy()})
+        ub, fo = [], []
+        for n in range(n_steps):
+            u_b, forcing = s.step_imex(dt)
+            ub.append(float(u_b))
+            fo.append(float(forcing))
+            for k in "uvwp":
+                d[f"step_{k}_{n + 1}"] = getattr(s, k).numpy().copy()
+        d["step_u_bulk"] = np.array(ub)
+        d["step_forcing"] = np.array(fo)
+    np.savez_compressed(os.path.join(HERE, "imex_small.npz"), **d)
+
+
+def gen_stats(ref):
+    torch, semilag, solver_mod, utils, projection, operators = ref
+    import contextlib
+    import io
+
+    from slchannel import turbstats
+
+    nx, ny, nz = 16, 12, 24
+    Lx, Ly, Lz, gamma = 4 * math.pi, 2 * math.pi, 2.0, 1.5
+    dx, dy = Lx / nx, Ly / ny
+    nu = 1.0 / 3000.0
+    z_f, z_c, dz_f, dz_c = utils.generate_grid(gamma, nz, Lz, stretching_type="symmetric")
+    zf, zc = z_f.numpy(), z_c.numpy()
+    with contextlib.redirect_stdout(io.StringIO()):
+        ts = turbstats.TurbulenceStats(nx, ny, nz, Lx, Ly, Lz, z_c, z_f, dz_c, dz_f, dx, dy,
+                                       nu, 180.0, z_plus_target=15.0, device="cpu",
+                                       top_wall_bc_type="dirichlet",
+                                       spectra_z_planes=[5.0, 40.0])
+    rng = np.random.default_rng(31)
+    d = {"nx": nx, "ny": ny, "nz": nz, "Lx": Lx, "Ly": Ly, "Lz": Lz, "gamma": gamma, "nu": nu,
+         "z_f": zf, "z_c": zc, "k_bot": ts.k_bot, "k_top": ts.k_top,
+         "extra_k": np.array([[pl["k_bot"], pl["k_top"]] for pl in ts.extra_planes])}
+    base = {"u": lambda X, Y, Z: 1.5 * (1.0 - (Z - 1.0) ** 2),
+            "v": lambda X, Y, Z: 0.05 + 0.0 * Z}
+    for n in range(3):
+        u, v, w = make_fields(rng, nx, ny, nz, dx, dy, Lx, Ly, Lz, zf, zc, base=base,
+                              amp=0.3, wamp=0.2)
+        tu, tv, tw = (torch.from_numpy(a.copy()) for a in (u, v, w))
+        solver_mod.apply_bc_all(tu, tv, tw, "dirichlet")
+        d[f"u{n}"], d[f"v{n}"], d[f"w{n}"] = (t.numpy().copy() for t in (tu, tv, tw))
+        ts.accumulate_statistics(tu, tv, tw, 0.05)
+    for name in ("U_sum", "uu_sum", "vv_sum", "ww_sum", "uw_sum", "Vm_sum", "Wm_sum",
+                 "Um2_sum", "Vm2_sum", "Wm2_sum", "UmWm_sum", "E_uu_2d_sum", "E_vv_2d_sum",
+                 "E_ww_2d_sum", "E_uw_2d_sum"):
+        d[name] = getattr(ts, name).numpy().copy()
+    for p, pl in enumerate(ts.extra_planes):
+        for c in ("uu", "vv", "ww", "uw"):
+            d[f"extra{p}_{c}"] = pl["sums"][c].numpy().copy()
+    with contextlib.redirect_stdout(io.StringIO()):
+        fin = ts.finalize_statistics()
+    for k in ("U_mean", "uu_mean", "vv_mean", "ww_mean", "uw_mean", "E_uu_2d", "E_vv_2d",
+              "E_ww_2d", "E_uw_2d", "u_tau", "n_samples"):
+        d[f"fin_{k}"] = np.asarray(fin[k])
+    for pl in ts.extra_planes:
+        for c in ("uu", "vv", "ww", "uw"):
+            key = f"E_{c}_2d_z{pl['zp']:.0f}"
+            d[f"fin_{key}"] = np.asarray(fin[key])
+    np.savez_compressed(os.path.join(HERE, "stats_small.npz"), **d)
+
+
+def gen_diag(ref):
+    torch, semilag, solver_mod, utils, projection, operators = ref
+    nx, ny, nz = 12, 10, 16
+    Lx, Ly, Lz, gamma = 4 * math.pi, 2 * math.pi, 2.0, 1.5
+    dx, dy = Lx / nx, Ly / ny
+    z_f, z_c, dz_f, dz_c = utils.generate_grid(gamma, nz, Lz, stretching_type="symmetric")
+    zf, zc = z_f.numpy(), z_c.numpy()
+    rng = np.random.default_rng(41)
+    base = {"u": lambda X, Y, Z: 1.5 * (1.0 - (Z - 1.0) ** 2)}
+    u, v, w = make_fields(rng, nx, ny, nz, dx, dy, Lx, Ly, Lz, zf, zc, base=base, amp=0.4,
+                          wamp=0.3)
+    tu, tv, tw = (torch.from_numpy(a.copy()) for a in (u, v, w))
+    solver_mod.apply_bc_all(tu, tv, tw, "dirichlet")
+    nu = 1.0 / 2800.0
+    d = {"nx": nx, "ny": ny, "nz": nz, "Lx": Lx, "Ly": Ly, "Lz": Lz, "gamma": gamma, "nu": nu,
+         "z_f": zf, "z_c": zc, "u": tu.numpy().copy(), "v": tv.numpy().copy(),
+         "w": tw.numpy().copy()}
+    d["u_tau_dirichlet"] = float(utils.compute_u_tau(tu, z_c, nu, "dirichlet"))
+    d["u_tau_neumann"] = float(utils.compute_u_tau(tu, z_c, nu, "neumann"))
+    d["dti"] = float(operators.compute_cfl_fused(tu, tv, tw, nx, ny, nz, dx, dy, dz_f, dz_c))
+    np.savez_compressed(os.path.join(HERE, "diag_small.npz"), **d)
+
+
 def main():
     ref_path = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     only = set(sys.argv[2:])
@@ -310,6 +481,9 @@ def main():
         ("advect_o6_traj4", lambda: gen_advect(ref, "o6_traj4", 6, seed=8, traj_order=4)),
         ("step_o4_traj4", lambda: gen_step(ref, "o4_traj4", 4, seed=9,
                                            sl_extra={"traj_interp_order": 4})),
+        ("imex_small", lambda: gen_imex(ref)),
+        ("stats_small", lambda: gen_stats(ref)),
+        ("diag_small", lambda: gen_diag(ref)),
     ]
     for name, make in cases:
         if not only or name in only:

@@ -155,3 +155,26 @@ def test_device_diagnostics_match_oracle(check):
     md = f.divergence().abs().max()
     check("SLChannelFlow.diagnostics max|div| == divergence()", row[0].item() == md.item(),
           f"{row[0].item()!r} vs {md.item()!r}")
+
+
+@pytest.mark.gpu
+def test_device_diagnostics_vs_reference_fixture(check, golden):
+    """slc_diagnostics against the reference's compute_u_tau (both top-wall
+    types) and compute_cfl_fused (tests/golden/diag_small.npz)."""
+    from slchannel_amd import HipPlan
+
+    g = golden("diag_small")
+    nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
+    u, v, w = (torch.tensor(g[c], device="cuda") for c in "uvw")
+    for top in ("dirichlet", "neumann"):
+        plan = HipPlan(nx, ny, nz, float(g["Lx"]), float(g["Ly"]), float(g["Lz"]), g["z_f"],
+                       g["z_c"], float(g["gamma"]), order=6, top_wall_bc_type=top,
+                       nu=float(g["nu"]), device="cuda:0")
+        out = torch.empty(5, dtype=torch.float64, device="cuda")
+        plan.diagnostics(u, v, w, None, 0.1, out)
+        got = out.tolist()
+        ref = float(g[f"u_tau_{top}"])
+        check(f"{top}: u_tau within 1e-15 of the reference", abs(got[1] - ref) <= 1e-15 * ref,
+              f"{got[1]!r} vs {ref!r}")
+        check(f"{top}: dti bit-identical to compute_cfl_fused", got[3] == float(g["dti"]),
+              f"{got[3]!r} vs {float(g['dti'])!r}")

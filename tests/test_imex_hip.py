@@ -69,3 +69,51 @@ def test_imex_steps_match_oracle(check, top):
         check(f"{top} step {n + 1}: u_bulk", abs(ub.item() - 1.0) < 1e-14, f"{ub.item()!r}")
     check(f"{top}: u, v, w, p vs oracle over 6 steps", worst < 1e-12,
           f"max rel={worst:.1e} per step={per} dt={dt:.4f}")
+
+
+@pytest.mark.parametrize("tag", ["sym", "bot"])
+def test_rhs_vs_reference_fixture(check, golden, tag):
+    """slc_imex_rhs against the reference's own compute_momentum_rhs_fused_imex
+    outputs (tests/golden/imex_small.npz): bit-identical."""
+    from slchannel_amd import HipPlan
+
+    g = golden("imex_small")
+    nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
+    top = str(g[f"{tag}_top"])
+    plan = HipPlan(nx, ny, nz, float(g["Lx"]), float(g["Ly"]), float(g["Lz"]), g[f"{tag}_z_f"],
+                   g[f"{tag}_z_c"], float(g["gamma"]),
+                   stretching_type="symmetric" if tag == "sym" else "bottom", order=6,
+                   top_wall_bc_type=top, nu=float(g[f"{tag}_nu"]), device="cuda:0")
+    u, v, w = (torch.tensor(g[f"{tag}_{c}"], device="cuda") for c in "uvw")
+    out = [torch.empty_like(t) for t in (u, v, w)]
+    plan.imex_rhs(u, v, w, *out)
+    for name, a in zip(("ru", "rv", "rw"), out):
+        a, b = a.cpu().numpy(), g[f"{tag}_{name}"]
+        check(f"{tag} {name} bit-identical to the reference", np.array_equal(a, b),
+              f"max|d|={np.abs(a - b).max():.1e} of {np.abs(b).max():.1e}")
+
+
+def test_imex_steps_vs_reference_fixture(check, golden):
+    """Three SLChannelFlow.step_imex steps against the reference's
+    (tests/golden/imex_small.npz): within 1e-12 (Thomas vs PCR, rocFFT vs
+    pocketfft rounding)."""
+    from slchannel_amd import SLChannelFlow
+
+    g = golden("imex_small")
+    nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
+    f = SLChannelFlow(config={"grid": {"nx": nx, "ny": ny, "nz": nz},
+                              "domain": {"Lx": float(g["Lx"]), "Ly": float(g["Ly"]),
+                                         "Lz": float(g["Lz"])},
+                              "flow": {"Re": float(g["step_Re"]), "U_bulk": 1.0,
+                                       "gamma": float(g["gamma"])},
+                              "sl": {"interp_order": 6}})
+    f.set_fields(g["step_u0"], g["step_v0"], g["step_w0"], g["step_p0"])
+    dt = float(g["step_dt"])
+    for n in range(int(g["step_n"])):
+        ub, fo = f.step_imex(dt)
+        check(f"step {n + 1} u_bulk", abs(ub.item() - g["step_u_bulk"][n]) < 1e-14,
+              f"{ub.item()!r}")
+        for nm in "uvwp":
+            a, b = getattr(f, nm).cpu().numpy(), g[f"step_{nm}_{n + 1}"]
+            e = float(np.abs(a - b).max() / np.abs(b).max())
+            check(f"step {n + 1} {nm} within 1e-12 of the reference", e < 1e-12, f"rel={e:.1e}")

@@ -154,3 +154,103 @@ def test_step_vs_reference(check, golden, name):
             for nm, a in (("u", co.u), ("v", co.v), ("w", co.w), ("p", co.p)):
                 e = rel_err(a, g[f"{nm}_{n + 1}"])
                 check(f"step {n + 1} {nm}", e < 1e-12, f"rel={e:.1e}")
+
+
+@pytest.mark.parametrize("tag", ["sym", "bot"])
+def test_imex_rhs_vs_reference(check, golden, tag):
+    """compute_momentum_rhs_fused_imex (operators.py:639-835): the oracle
+    restates the reference's element operations in order, so bit-identical."""
+    g = golden("imex_small")
+    nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
+    z_f, z_c = g[f"{tag}_z_f"], g[f"{tag}_z_c"]
+    dz_f, dz_c = z_f[1:] - z_f[:-1], z_c[1:] - z_c[:-1]
+    dx, dy = float(g["Lx"]) / nx, float(g["Ly"]) / ny
+    out = orc.compute_momentum_rhs_fused_imex(g[f"{tag}_u"], g[f"{tag}_v"], g[f"{tag}_w"],
+                                              nx, ny, nz, dx, dy, dz_c, dz_f,
+                                              float(g[f"{tag}_nu"]))
+    for name, a in zip(("ru", "rv", "rw"), out):
+        b = g[f"{tag}_{name}"]
+        check(f"{tag} {name} bit-identical to the reference", np.array_equal(a, b),
+              f"max|d|={np.abs(a - b).max():.1e} of {np.abs(b).max():.1e}")
+
+
+def test_imex_steps_vs_reference(check, golden):
+    """SLChannelFlow.step_imex (solver.py:1055-1134): three steps, forward
+    Euler then AB2, Crank-Nicolson z-solves, projection, bulk shift."""
+    g = golden("imex_small")
+    nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
+    co = orc.ChannelOracle(nx, ny, nz, float(g["Lx"]), float(g["Ly"]), float(g["Lz"]),
+                           float(g["gamma"]), float(g["step_Re"]), order=6,
+                           z_f=g["step_z_f"], z_c=g["step_z_c"])
+    co.set_fields(g["step_u0"], g["step_v0"], g["step_w0"], g["step_p0"])
+    dt = float(g["step_dt"])
+    for n in range(int(g["step_n"])):
+        ub, fo = co.step_imex(dt)
+        check(f"step {n + 1} u_bulk", abs(ub - g["step_u_bulk"][n]) < 1e-14, f"{ub!r}")
+        check(f"step {n + 1} forcing",
+              abs(fo - g["step_forcing"][n]) < 1e-12 * max(1.0, abs(fo)),
+              f"{fo:.6e} vs {g['step_forcing'][n]:.6e}")
+        for nm in "uvwp":
+            e = rel_err(getattr(co, nm), g[f"step_{nm}_{n + 1}"])
+            check(f"step {n + 1} {nm}", e < 1e-12, f"rel={e:.1e}")
+
+
+def test_stats_vs_reference(check, golden):
+    """TurbulenceStats.accumulate_statistics over three snapshots with two
+    extra spectra planes, and finalize_statistics (turbstats.py:156-467)."""
+    from oracle.turbstats import PROFILE_NAMES, StatsOracle
+
+    g = golden("stats_small")
+    nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
+    planes = [(int(g["k_bot"]), int(g["k_top"]))] + [tuple(int(x) for x in r)
+                                                      for r in g["extra_k"]]
+    st = StatsOracle(nx, ny, nz, planes=planes)
+    for n in range(3):
+        st.accumulate(g[f"u{n}"], g[f"v{n}"], g[f"w{n}"])
+    for i, name in enumerate(PROFILE_NAMES):
+        e = rel_err(st.prof[i], g[name])
+        check(f"{name} within 1e-14 of the reference", e < 1e-14, f"rel={e:.1e}")
+    for c, comp in enumerate(("uu", "vv", "ww", "uw")):
+        e = rel_err(st.spec[0][c], g[f"E_{comp}_2d_sum"])
+        check(f"E_{comp}_2d_sum within 1e-14", e < 1e-14, f"rel={e:.1e}")
+        for p in range(len(planes) - 1):
+            e = rel_err(st.spec[p + 1][c], g[f"extra{p}_{comp}"])
+            check(f"extra plane {p} {comp} within 1e-14", e < 1e-14, f"rel={e:.1e}")
+    fin = st.finalize(float(g["nu"]), float(g["z_c"][1]))
+    e = rel_err(fin["U_mean"], g["fin_U_mean"])
+    check("finalized U_mean within 1e-14", e < 1e-14, f"rel={e:.1e}")
+    # the stresses are sum/n + (squared-mean sum)/n - mean^2: a cancelling
+    # difference, so their rounding is measured on the scale of the squared
+    # plane means (the largest second moment), as the plane-mean sums of the
+    # two libraries differ in their last bits
+    scale2 = float(np.abs(g["Um2_sum"]).max()) / 3.0
+    for k in ("uu_mean", "vv_mean", "ww_mean", "uw_mean"):
+        e = float(np.abs(fin[k] - g[f"fin_{k}"]).max()) / scale2
+        check(f"finalized {k} within 1e-14 of the second-moment scale", e < 1e-14,
+              f"rel={e:.1e} (scale {scale2:.2f})")
+    for c, comp in enumerate(("uu", "vv", "ww", "uw")):
+        e = rel_err(fin["spectra"][0][c], g[f"fin_E_{comp}_2d"])
+        check(f"finalized E_{comp}_2d within 1e-14", e < 1e-14, f"rel={e:.1e}")
+    check("u_tau within 1e-15", abs(fin["u_tau"] - float(g["fin_u_tau"]))
+          < 1e-15 * float(g["fin_u_tau"]),
+          f"{fin['u_tau']!r} vs {float(g['fin_u_tau'])!r}")
+
+
+def test_diagnostics_vs_reference(check, golden):
+    """compute_u_tau (utils.py:116-146) both top-wall types and
+    compute_cfl_fused (operators.py:839-940)."""
+    g = golden("diag_small")
+    nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
+    z_f, z_c = g["z_f"], g["z_c"]
+    dz_f, dz_c = z_f[1:] - z_f[:-1], z_c[1:] - z_c[:-1]
+    dx, dy = float(g["Lx"]) / nx, float(g["Ly"]) / ny
+    nu = float(g["nu"])
+    for top in ("dirichlet", "neumann"):
+        ut = orc.compute_u_tau(g["u"], z_c, nu, top)
+        ref = float(g[f"u_tau_{top}"])
+        # the plane mean of the first level sums in numpy's order, not torch's
+        check(f"u_tau ({top}) within 1e-15", abs(ut - ref) <= 1e-15 * ref,
+              f"{ut!r} vs {ref!r}")
+    dti = orc.compute_cfl_fused(g["u"], g["v"], g["w"], nx, ny, nz, dx, dy, dz_f, dz_c)
+    check("compute_cfl_fused bit-identical", dti == float(g["dti"]),
+          f"{dti!r} vs {float(g['dti'])!r}")

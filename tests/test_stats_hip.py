@@ -184,3 +184,38 @@ def test_state_round_trip_and_determinism(check, tmp_path):
     except ValueError:
         ok = True
     check("grid mismatch rejected", ok, "")
+
+
+def test_reference_fixture(check, golden):
+    """TurbulenceStats on the GPU against the reference's own accumulate and
+    finalize outputs (tests/golden/stats_small.npz: three snapshots, the
+    primary and two extra spectra planes), to the same bar as the oracle."""
+    g = golden("stats_small")
+    nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
+    ts, zc1 = _stats(nx, ny, nz, planes=[5.0, 40.0])
+    check("plane indices == reference", (ts.k_bot, ts.k_top) == (int(g["k_bot"]), int(g["k_top"]))
+          and [[p["k_bot"], p["k_top"]] for p in ts.extra_planes] == g["extra_k"].tolist(),
+          f"{ts.k_bot, ts.k_top}")
+    for n in range(3):
+        ts.accumulate_statistics(*(torch.tensor(g[f"{c}{n}"], device="cuda") for c in "uvw"),
+                                 0.05)
+    names = ("U_sum", "uu_sum", "vv_sum", "ww_sum", "uw_sum", "Vm_sum", "Wm_sum", "Um2_sum",
+             "Vm2_sum", "Wm2_sum", "UmWm_sum")
+    s1 = max(np.max(np.abs(g[names[q]])) for q in FIRST)
+    s2 = max(np.max(np.abs(g[n])) for q, n in enumerate(names) if q not in FIRST)
+    worst = max((_rel(getattr(ts, n).cpu().numpy(), g[n], s1 if q in FIRST else s2), n)
+                for q, n in enumerate(names))
+    check(f"profile sums within {TOL:g} of the reference", worst[0] < TOL,
+          f"worst {worst[1]} {worst[0]:.1e}")
+    got = ts.finalize_statistics()
+    ref = {k: g[f"fin_{k}"] for k in ("U_mean", "uu_mean", "vv_mean", "ww_mean", "uw_mean")}
+    _compare_final(check, "reference", got, ref)
+    check("u_tau", abs(got["u_tau"] - float(g["fin_u_tau"])) <= TOL * float(g["fin_u_tau"]),
+          f"{got['u_tau']!r} vs {float(g['fin_u_tau'])!r}")
+    for comp in ("uu", "vv", "ww", "uw"):
+        r = _rel(got[f"E_{comp}_2d"], g[f"fin_E_{comp}_2d"])
+        check(f"finalized E_{comp}_2d within {TOL:g}", r < TOL, f"{r:.1e}")
+        for zp in (5, 40):
+            key = f"E_{comp}_2d_z{zp}"
+            r = _rel(got[key], g[f"fin_{key}"])
+            check(f"finalized {key} within {TOL:g}", r < TOL, f"{r:.1e}")

@@ -9,8 +9,15 @@
   checked against the flow's own amplification, measured by a second oracle
   run started one ulp away: the device's difference to the oracle stays
   within 100x of it at steps 10..50.  Every step: u_bulk = 1 to 1e-14, and
-  max|div| < 1e-11 or no larger than the oracle's own (the projection
-  residual grows with the field once the transient strengthens).
+  max|div| below the projection's rounding floor.  That floor is the
+  backward error of the Poisson solve, eps * dt_eff * max|phi| * max_k
+  2 / (dz_c[k] dz_f[k]): it grows with the pressure, which grows 55x over
+  this transient (max|phi| 0.21 -> 12), so the oracle itself -- the
+  reference's algorithm in FP64 -- exceeds 1e-11 from step 32 on (1.6e-11),
+  at 0.3-3.6x that floor on every one of the 50 steps.  The bar is 1e-11 or
+  8x the floor, whichever is larger; part of the residual is the (0, 0)
+  mode at the first z plane, whose equation the identity-row pin of
+  projection.py:113-114 replaces.
 * 256^3, order 6, Re_b 2792.8, k^-5/3 perturbation (u'rms 0.15,
   1 <= |k| <= 32), dt+ = 0.2 -- the benchmark workload: the BDF1 bootstrap
   and one BDF2 step, velocities within 1e-12 of the oracle (observed
@@ -18,7 +25,21 @@
   1.7e-12): it solves a Poisson problem whose right-hand side, the
   divergence of the predictor over dt_eff, is a small difference of O(1)
   terms, so rocFFT/pocketfft rounding shows in it at a larger relative
-  size than in the projected velocities.
+  size than in the projected velocities.  Both interpolation orders (6 and
+  4).  Then the workload's 200-step horizon at order 6: every step
+  max|div| under the bar above, u_bulk = 1 to 1e-14, finite fields.
+* 1024x1024x512 (Lx 8 pi, Ly 3 pi, tanh 1.8, Re_b 20000, order 6, k^-5/3
+  perturbation u'rms 0.12, trajectory CFL 4): three steps, every step
+  max|div| under the same bar, u_bulk = 1 to 1e-14, finite fields, clamp
+  count >= 0
+  (the oracle does not finish a step of this size in a test's time).
+* The 48-member uneven ensemble (grids drawn from {64x48x64, 96x64x96,
+  128x96x128, 192x128x128}, Re U[2800, 6000], seeds 4000 + m, order 6,
+  CFL 3): three batched steps, every member bit-identical to its own
+  SLChannelFlow run (fields, pressure, bulk velocity, forcing, clamps).
+* A served stream over the four serving grids (64^3, 128x96x128, 256^3,
+  384x384x256; Re in [2800, 10000], 3-10 steps, staggered arrivals): every
+  result bit-identical to the same request run alone on the plain path.
 """
 
 import math
@@ -51,6 +72,17 @@ def _pair(n, nz, Re, order, seed, urms, kmax, spectrum="flat", kmin=1.0):
     return f, co
 
 
+def _div_floor(f, dt_eff):
+    """eps * dt_eff * max|phi| * max_k 2 / (dz_c[k] dz_f[k]): the divergence a
+    backward-stable Poisson solve leaves after the projection (phi = p, the
+    'noninc' pressure; the largest coefficient of the z operator is at the
+    walls)."""
+    dz_c = f.dz_c.cpu().numpy()
+    dz_f = f.dz_f.cpu().numpy()
+    a_norm = float((2.0 / (dz_c[:-1] * dz_f)).max())
+    return float(np.finfo(np.float64).eps) * dt_eff * f.p.abs().max().item() * a_norm
+
+
 def test_reference_case_64cubed_50_steps(check):
     from slchannel_amd.diagnostics import cfl_dt
 
@@ -74,15 +106,18 @@ def test_reference_case_64cubed_50_steps(check):
         md = f.divergence().abs().max().item()
         od = float(np.abs(orc.compute_divergence(co.u, co.v, co.w, 64, 64, 64, co.dx, co.dy,
                                                  co.dz_f)).max())
-        divs.append((md, od))
+        dt_eff = dt if n == 0 else (2.0 / 3.0) * dt
+        divs.append((md, od, _div_floor(f, dt_eff)))
         ncl, ocl = int(f.sl.n_clamped_last.item()), int(co.n_clamped_last)
         if abs(ub.item() - 1.0) >= 1e-14 or (n < 20 and ncl != ocl):
             bad.append((n + 1, ub.item(), ncl, ocl))
     check("u_bulk = 1 every step; clamps == oracle while the runs agree", not bad, f"{bad[:3]}")
-    worst_div = max(d[0] for d in divs)
-    check("max|div| < 1e-11 every step, or no larger than the oracle's own",
-          all(m < 1e-11 or m <= 2.0 * o for m, o in divs),
-          f"max {worst_div:.1e}; step 33 device {divs[32][0]:.1e} oracle {divs[32][1]:.1e}")
+    worst = max(range(len(divs)), key=lambda k: divs[k][0] / divs[k][2])
+    check("max|div| < max(1e-11, 8 x the Poisson solve's rounding floor) every step",
+          all(m < max(1e-11, 8.0 * fl) for m, _, fl in divs),
+          f"max {max(d[0] for d in divs):.1e}; worst ratio to the floor "
+          f"{divs[worst][0] / divs[worst][2]:.1f} at step {worst + 1} "
+          f"(device {divs[worst][0]:.1e}, oracle {divs[worst][1]:.1e}, floor {divs[worst][2]:.1e})")
     check("first 10 steps within 1e-12 of the oracle", max(errs[:10]) < 1e-12,
           f"{[f'{e:.0e}' for e in errs[:10]]} dt={dt:.4f}")
     marks = (9, 19, 29, 39, 49)
@@ -92,8 +127,10 @@ def test_reference_case_64cubed_50_steps(check):
           + " | oracle vs oracle+1ulp " + ", ".join(f"{k + 1}: {ulp[k]:.1e}" for k in marks))
 
 
-def test_benchmark_workload_256cubed_two_steps(check):
-    f, co = _pair(256, 256, 2792.8, 6, 2000, 0.15, 32.0, spectrum="k-5/3")
+@pytest.mark.parametrize("order", [6, 4])
+def test_benchmark_workload_256cubed_two_steps(check, order):
+    print(f"256^3 order {order}: BDF1 + BDF2 against the oracle", flush=True)
+    f, co = _pair(256, 256, 2792.8, order, 2000, 0.15, 32.0, spectrum="k-5/3")
     nu = f.nu
     u_tau = 180.0 * nu
     dt = 0.2 * nu / (u_tau * u_tau)
@@ -101,11 +138,160 @@ def test_benchmark_workload_256cubed_two_steps(check):
         ub, _ = f.step_sl_bdf2(dt)
         co.step_sl_bdf2(dt)
         e = {k: _rel(getattr(f, k), getattr(co, k)) for k in "uvwp"}
-        check(f"256^3 {name}: u, v, w within 1e-12, p within 1e-11 of the oracle",
+        check(f"256^3 o{order} {name}: u, v, w within 1e-12, p within 1e-11 of the oracle",
               max(e["u"], e["v"], e["w"]) < 1e-12 and e["p"] < 1e-11,
               " ".join(f"{k} {x:.1e}" for k, x in e.items()))
         md = f.divergence().abs().max().item()
-        check(f"256^3 {name}: max|div| < 1e-11, u_bulk = 1", md < 1e-11
+        check(f"256^3 o{order} {name}: max|div| < 1e-11, u_bulk = 1", md < 1e-11
               and abs(ub.item() - 1.0) < 1e-14, f"{md:.1e} {ub.item()!r}")
-        check(f"256^3 {name}: clamps == oracle", int(f.sl.n_clamped_last.item())
+        check(f"256^3 o{order} {name}: clamps == oracle", int(f.sl.n_clamped_last.item())
               == int(co.n_clamped_last), f"{int(f.sl.n_clamped_last.item())} vs {co.n_clamped_last}")
+
+
+def _flow(nx, ny, nz, Lx, Ly, gamma, Re, seed, urms, kmax, spectrum):
+    from slchannel_amd import SLChannelFlow
+
+    cfg = {"grid": {"nx": nx, "ny": ny, "nz": nz},
+           "domain": {"Lx": Lx, "Ly": Ly, "Lz": 2.0},
+           "flow": {"Re": Re, "U_bulk": 1.0, "gamma": gamma},
+           "sl": {"interp_order": 6}}
+    f = SLChannelFlow(config=cfg)
+    f.initialize(seed=seed, urms=urms, kmin=1.0, kmax=kmax, spectrum=spectrum)
+    return f
+
+
+def _properties(f, dt, steps, label, check):
+    """Step `steps` times; every step max|div| < max(1e-11, 8 x the Poisson
+    solve's rounding floor), u_bulk = 1 to 1e-14, no non-finite entry, clamp
+    count >= 0.  Also reports on how many steps the absolute 1e-11 held."""
+    import torch
+
+    bad, divs, ratios = [], [], []
+    diag = torch.empty(5, dtype=torch.float64, device=f.u.device)
+    for n in range(steps):
+        ub, _ = f.step_sl_bdf2(dt)
+        f.plan.diagnostics(f.u, f.v, f.w, f.p, dt, diag)
+        d = diag.tolist()
+        ncl = int(f.sl.n_clamped_last.item())
+        floor = _div_floor(f, dt if n == 0 else (2.0 / 3.0) * dt)
+        divs.append(d[0])
+        ratios.append(d[0] / floor)
+        if not (d[0] < max(1e-11, 8.0 * floor) and abs(ub.item() - 1.0) < 1e-14 and d[4] == 0
+                and ncl >= 0):
+            bad.append((n + 1, d[0], floor, ub.item(), d[4], ncl))
+    held = sum(x < 1e-11 for x in divs)
+    check(f"{label}: every step max|div| < max(1e-11, 8 x floor), u_bulk = 1, finite, "
+          "clamps >= 0", not bad,
+          f"max|div| {max(divs):.1e}, max ratio to the floor {max(ratios):.1f}, "
+          f"< 1e-11 on {held}/{steps} steps; failures {bad[:3]}")
+
+
+def test_benchmark_workload_256cubed_200_steps(check):
+    print("256^3 order 6: 200-step horizon", flush=True)
+    f = _flow(256, 256, 256, 4 * math.pi, 2 * math.pi, 1.5, 2792.8, 2000, 0.15, 32.0, "k-5/3")
+    u_tau = 180.0 * f.nu
+    dt = 0.2 * f.nu / (u_tau * u_tau)
+    _properties(f, dt, 200, "256^3 order 6, 200 steps", check)
+
+
+def test_large_box_1024x1024x512_three_steps(check):
+    import torch
+
+    from slchannel_amd.diagnostics import cfl_dt
+
+    print("1024x1024x512 order 6: three steps", flush=True)
+    f = _flow(1024, 1024, 512, 8 * math.pi, 3 * math.pi, 1.8, 20000.0, 3000, 0.12, 32.0,
+              "k-5/3")
+    dt = cfl_dt(f, 4.0)
+    _properties(f, dt, 3, f"1024x1024x512 (dt {dt:.4f}, CFL 4)", check)
+    del f
+    torch.cuda.empty_cache()
+
+
+ENSEMBLE_GRIDS = ((64, 48, 64), (96, 64, 96), (128, 96, 128), (192, 128, 128))
+
+
+def _member_cfg(nx, ny, nz, Re):
+    return {"grid": {"nx": nx, "ny": ny, "nz": nz},
+            "domain": {"Lx": 4 * math.pi, "Ly": 2 * math.pi, "Lz": 2.0},
+            "flow": {"Re": Re, "U_bulk": 1.0, "gamma": 1.5},
+            "sl": {"interp_order": 6}}
+
+
+def test_ensemble_48_uneven_members_bit_identical(check):
+    import torch
+
+    from slchannel_amd import SLChannelFlow
+    from slchannel_amd.diagnostics import cfl_dt
+    from slchannel_amd.ensemble import Ensemble
+
+    print("48-member ensemble: three steps against each member's own run", flush=True)
+    rng = np.random.Generator(np.random.PCG64(4000))
+    specs = []
+    for m in range(48):
+        nx, ny, nz = ENSEMBLE_GRIDS[int(rng.integers(len(ENSEMBLE_GRIDS)))]
+        specs.append((nx, ny, nz, float(rng.uniform(2800.0, 6000.0)), 4000 + m))
+    inits, dts, alone = [], [], []
+    for nx, ny, nz, Re, seed in specs:
+        f = SLChannelFlow(config=_member_cfg(nx, ny, nz, Re))
+        f.initialize(seed=seed, urms=0.1, kmax=8.0)
+        dt = cfl_dt(f, 3.0)
+        inits.append((f.u.clone(), f.v.clone(), f.w.clone(), f.p.clone()))
+        dts.append(dt)
+        for _ in range(3):
+            ub, fo = f.step_sl_bdf2(dt)
+        alone.append(({k: getattr(f, k).clone() for k in "uvwp"},
+                      [ub.item(), fo.item(), float(f.sl.n_clamped_last.item())]))
+        del f
+    ens = Ensemble(order=6)
+    ids = [ens.add(_member_cfg(nx, ny, nz, Re), dt, fields=ini)
+           for (nx, ny, nz, Re, _), dt, ini in zip(specs, dts, inits)]
+    for _ in range(3):
+        ens.step()
+    diff = []
+    for m, (mid, (fields, stats)) in enumerate(zip(ids, alone)):
+        got = ens.get_fields(mid)
+        st = ens.stats(mid).tolist()[:3]
+        if not (all(torch.equal(got[k], fields[k]) for k in "uvwp") and st == stats):
+            diff.append((m, specs[m][:3], max((got[k] - fields[k]).abs().max().item()
+                                               for k in "uvwp")))
+    grids = sorted({s[:3] for s in specs})
+    check("48 members, 3 steps: every member bit-identical to its own run", not diff,
+          f"grids {grids}; differing members {diff[:4]}")
+    ens.close()
+    torch.cuda.empty_cache()
+
+
+def test_served_stream_over_the_serving_grids(check):
+    import torch
+
+    from slchannel_amd.serve import Request, Server, run_alone
+
+    print("served stream over 64^3, 128x96x128, 256^3, 384x384x256", flush=True)
+    plan = ((64, 64, 64, 2800.0, 8, 0.0), (256, 256, 256, 6000.0, 5, 0.0),
+            (128, 96, 128, 4200.0, 10, 0.01), (384, 384, 256, 9000.0, 4, 0.02),
+            (64, 64, 64, 10000.0, 6, 0.05), (128, 96, 128, 3100.0, 3, 0.08))
+    reqs = [Request(rid=i, config=_member_cfg(nx, ny, nz, Re), n_steps=n, seed=5000 + i,
+                    arrival=t) for i, (nx, ny, nz, Re, n, t) in enumerate(plan)]
+    srv = Server(keep_fields=True)
+    srv.warmup([r.config for r in reqs])
+    results, metrics = srv.run(reqs)
+    check("every request served", sorted(results) == list(range(len(reqs))) and
+          all(r.status == "ok" for r in results.values()),
+          f"{[(k, r.status) for k, r in sorted(results.items())]}")
+    joined_mid = any(0 < r.admitted < max(x.finished for x in results.values())
+                     for r in results.values())
+    check("runs joined while others were mid-flight", joined_mid, "")
+    for r in reqs:
+        alone, _ = run_alone(r)
+        got = results[r.rid]
+        same = all(torch.equal(got.fields[k], alone.fields[k]) for k in "uvwp")
+        c = r.config["grid"]
+        check(f"request {r.rid} ({c['nx']}x{c['ny']}x{c['nz']}, {r.n_steps} steps): "
+              "mix == alone", same and got.stats[:3] == alone.stats[:3],
+              f"max|d|={max((got.fields[k] - alone.fields[k]).abs().max().item() for k in 'uvwp'):.1e}")
+        del alone
+    check("latency percentiles", metrics["p99_step_ms"] >= metrics["p50_step_ms"] > 0,
+          f"p50 {metrics['p50_step_ms']:.2f} ms p99 {metrics['p99_step_ms']:.2f} ms")
+    del srv, results
+    torch.cuda.empty_cache()
