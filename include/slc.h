@@ -221,6 +221,17 @@ int slc_diagnostics(slc_plan* plan, const double* u, const double* v, const doub
 int slc_bulk_force(slc_plan* plan, double* u, double* v, double* w, double U_bulk, double dt,
                    double* out, void* stream);
 
+/* slc_bulk_force without the plan's workspace: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * the call allocates nothing.  Its partial sums (at most 2048 doubles, the
+ * most any grid needs) live in a static device array of the library, one per
+ * GPU, held until the process ends.  Calls that overlap on one GPU share that
+ * array and race: order them on one stream.  Same kernels, same results as
+ * slc_bulk_force. */
+int slc_bulk_force_noalloc(slc_plan* plan, double* u, double* v, double* w, double U_bulk,
+                           double dt, double* out, void* stream);
+
 /* Options.
  * SLC_OPT_GATHER_TILE (no counterpart in the reference): -1 = direct
  * global-load kernels; 0 = fused LDS-tiled step kernels (4x4x16 arrival

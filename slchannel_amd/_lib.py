@@ -20,7 +20,8 @@ EXPORTS = (
     "slc_stencil", "slc_locate_z", "slc_advect", "slc_bdf1_predict", "slc_bdf2_predict",
     "slc_zsolve", "slc_divergence", "slc_poisson", "slc_project", "slc_project_step",
     "slc_pressure_combine",
-    "slc_bulk_velocity", "slc_bulk_force", "slc_diagnostics", "slc_imex_rhs",
+    "slc_bulk_velocity", "slc_bulk_force", "slc_bulk_force_noalloc", "slc_diagnostics",
+    "slc_imex_rhs",
     "slc_ab2_update", "slc_set_option",
     "slc_profile_enable", "slc_profile_report",
     "slc_ens_create", "slc_ens_destroy", "slc_ens_member_bytes", "slc_ens_add",
@@ -85,6 +86,7 @@ def load():
         "slc_project_step": ([vp, vp, vp, vp, d, d, vp, vp], i),
         "slc_bulk_velocity": ([vp, vp, vp, vp], i),
         "slc_bulk_force": ([vp, vp, vp, vp, d, d, vp, vp], i),
+        "slc_bulk_force_noalloc": ([vp, vp, vp, vp, d, d, vp, vp], i),
         "slc_diagnostics": ([vp, vp, vp, vp, vp, d, vp, vp], i),
         "slc_imex_rhs": ([vp, vp, vp, vp, vp, vp, vp, vp], i),
         "slc_ab2_update": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, d, vp], i),

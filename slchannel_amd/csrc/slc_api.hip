@@ -345,6 +345,44 @@ int slc::plan_create(const slc_grid_desc* d, int device, bool spectral, slc_plan
     return SLC_OK;
 }
 
+// ---------------------------------------------------------------------------
+// slc_bulk_force_noalloc: the bulk-flux shift with its partial sums in a
+// file-scope device array instead of the plan's workspace.  Each device holds
+// its own copy of `scratch`; its address on a device is looked up once, with
+// that device current, and kept in g_scratch_dev[].  Host code uses only that
+// address: the array's name in host code is not a device address.
+// ---------------------------------------------------------------------------
+
+namespace {
+constexpr int kNoAllocPartials = 2048;  // the most bulk_partials_needed() returns
+constexpr int kNoAllocDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kNoAllocDevices] = {};
+
+// The device address of `scratch` on `device` (current here), looked up on
+// the first call for that device.
+int scratch_on(int device, double** out) {
+    if (device < 0 || device >= kNoAllocDevices)
+        return fail(SLC_EINVAL, "device %d outside 0..%d", device, kNoAllocDevices - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(SLC_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                        hipGetErrorString(e));
+        g_scratch_dev[device] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_dev[device];
+    return SLC_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int slc_version(void) { return 100; }
@@ -661,6 +699,24 @@ int slc_bulk_force(slc_plan* p, double* u, double* v, double* w, double U_bulk, 
     Timed t(p, "bulk_force", stream);
     HIP_OK(run_bulk(p->g, u, v, w, p->d_cv, p->total_volume, U_bulk, dt, 1, p->partials,
                     p->max_partials, out, S(stream)));
+    return SLC_OK;
+}
+
+int slc_bulk_force_noalloc(slc_plan* p, double* u, double* v, double* w, double U_bulk,
+                           double dt, double* out, void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (!u || !out) return fail(SLC_EINVAL, "null array");
+    if (!(dt != 0.0)) return fail(SLC_EINVAL, "dt must be non-zero");
+    const int np = bulk_partials_needed(p->g);
+    if (np > kNoAllocPartials)
+        return fail(SLC_EINVAL, "%d bulk partial sums exceed the static scratch of %d", np,
+                    kNoAllocPartials);
+    DeviceGuard dg(p->device);
+    double* partials = nullptr;
+    if (int rc = scratch_on(p->device, &partials)) return rc;
+    // every partial is written before it is read: nothing to clear
+    HIP_OK(run_bulk(p->g, u, v, w, p->d_cv, p->total_volume, U_bulk, dt, 1, partials,
+                    kNoAllocPartials, out, S(stream)));
     return SLC_OK;
 }
 
