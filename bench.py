@@ -10,11 +10,14 @@ full SL-BDF2 step (predictor, z-solves, FFT projection, bulk shift); the W
 warm-up steps include the BDF1 bootstrap.  Inputs are resident in HBM.
 
 Prints ONE JSON line.  Also reported:
-  roofline      the dominant kernel -- the far-foot BDF2 pass of u (two-foot
-                tracer + order-P gather) -- timed with hipEvents on its launch
-                stream over the timed region; algorithmic bytes per arrival
-                point: U* 3x8 + V^{n-1} 8 read, near foot 3x8 + out 8 written
-                = 64 B;
+  roofline      the dominant kernel -- the fused BDF2 pass of u (two-foot
+                tracer, V^{n-1} gathered at the 2dt foot and V^n at the dt
+                foot, xy right-hand side, BDF2 combination) -- timed with
+                hipEvents on its launch stream over the timed region;
+                algorithmic bytes per arrival point: U* 3x8 + V^{n-1} 8 +
+                V^n 8 read, the prediction 8 written = 48 B (--bdf2-split:
+                the far pass, 64 B); checked against the FP64 vector peak
+                too (fp64_check);
   gather_GBps   the isolated order-P gather of u (slc_gather) at 40 B/point
                 (3x8 B feet + 8 B field + 8 B out), feet from the 2dt foot;
   cpu_baseline  the CPU oracle (oracle/: numpy + OpenMP C) timed on this
@@ -55,12 +58,15 @@ def parse():
     ap.add_argument("--bdf2-split", action="store_true",
                     help="far pass, then near pass, instead of the fused BDF2 pass "
                          "(same doubles; for profiling the two passes)")
-    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5, 6),
+    ap.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5, 6, 7),
                     help="2: 256^3 channel (default, the headline workload); "
                          "3: 1024x1024x512 large box, CFL 4, plus the isolated gather "
                          "with feet displaced uniformly in +-5 cells; "
                          "4: 48-member uneven ensemble, batched vs one run at a time; "
-                         "5: serving loop with Poisson arrivals (p50/p99 step latency)")
+                         "5: serving loop with Poisson arrivals (p50/p99 step latency); "
+                         "6: on-device turbulence statistics; "
+                         "7: the reference's advect microbenchmark (three-component "
+                         "SLAdvector.advect at 768x768x180, orders 4 and 6)")
     ap.add_argument("--scheme", default="sl", choices=("sl", "imex"),
                     help="config 2: time the SL-BDF2 step (default) or the Eulerian IMEX "
                          "step at the reference's Eulerian CFL 0.28, with both schemes' "
@@ -104,6 +110,8 @@ def main():
         return serve_bench(args)
     if args.config == 6:
         return stats_bench(args)
+    if args.config == 7:
+        return advect_bench(args)
     n, nz = args.n, args.nz or args.n
     Re_b, Re_tau, dtp = 2792.8, 180.0, 0.2
     cfg = {
@@ -631,6 +639,68 @@ def serve_bench(args):
         "plain_p99_time_in_system_s": round(pf["p99_time_in_system_s"], 3),
         "plain_back_to_back_member_steps_per_s": round(total_steps / sum(durations), 2),
         "served_vs_plain": round(m["member_steps_per_s"] / plain, 3),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def advect_bench(args):
+    """The reference's published advect microbenchmark
+    (benchmarks/bench_interp.py:34-111): the three-component
+    SLAdvector.advect(u, v, w, u, v, w, dt) -- departure feet plus the
+    order-P gather of every component -- at 768 x 768 x 180 (Lx 15.707,
+    Ly 6.282, Lz 2, tanh 2.6), u = 1 + 0.1 U(0,1), v = 0.2 U(0,1),
+    w = 0.1 U(0,1), dt 0.05; 3 warm-up calls, then the mean of 5.  The
+    reference publishes 126 ms (order 4) and 143 ms (order 6) for its Triton
+    fp32 pipeline on a GB10 (report/sl_dns_report.tex:765); here FP64 and the
+    FP32-storage variant (FP64 weights and accumulation)."""
+    import torch
+
+    from slchannel_amd import SLAdvector
+    from slchannel_amd.grid import generate_grid
+
+    nx, ny, nz = 768, 768, 180
+    Lx, Ly, Lz, gamma = 15.707, 6.282, 2.0, 2.6
+    z_f, z_c, _, _ = generate_grid(gamma, nz, Lz)
+    g = torch.Generator(device="cuda").manual_seed(7)
+
+    def rand(*shape):
+        return torch.rand(*shape, generator=g, device="cuda", dtype=torch.float64)
+
+    u = 1.0 + 0.1 * rand(nx + 1, ny + 2, nz + 2)
+    v = 0.2 * rand(nx + 2, ny + 1, nz + 2)
+    w = 0.1 * rand(nx + 2, ny + 2, nz + 1)
+    dt = torch.tensor(0.05, dtype=torch.float64)
+    published = {4: 126.0, 6: 143.0}
+    rows = []
+    for order in (4, 6):
+        for dtype in ("fp64", "fp32_accum64"):
+            adv = SLAdvector(nx, ny, nz, Lx / nx, Ly / ny, Lx, Ly, Lz, z_f, z_c, gamma,
+                             order=order, interp_dtype=dtype, device="cuda:0")
+            for _ in range(3):
+                adv.advect(u, v, w, u, v, w, dt)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(5):
+                adv.advect(u, v, w, u, v, w, dt)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) / 5 * 1e3
+            rows.append({"order": order, "interp_dtype": dtype, "ms": round(ms, 3),
+                         "published_gb10_triton_fp32_ms": published[order]})
+            del adv
+            torch.cuda.empty_cache()
+    best6 = min(r["ms"] for r in rows if r["order"] == 6 and r["interp_dtype"] == "fp64")
+    line = {
+        "metric": "ms per three-component advect (768x768x180, bench_interp.py workload)",
+        "value": best6, "unit": "ms", "n_gpus": 1, "steps": 5, "warmup": 3,
+        "ms_per_step": best6, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (the reference benchmark's uniform random fields)",
+        "config": {"workload": "SLAdvector.advect 768x768x180, Lx 15.707, Ly 6.282, "
+                               "tanh 2.6, dt 0.05", "nx": nx, "ny": ny, "nz": nz},
+        "cases": rows,
+        "note": "the published figures are the reference's Triton fp32 pipeline on a GB10; "
+                "context, not a like-for-like target",
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)

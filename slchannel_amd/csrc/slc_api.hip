@@ -199,6 +199,9 @@ int slc::plan_create(const slc_grid_desc* d, int device, bool spectral, slc_plan
     g.Lz = d->Lz;
     g.gamma = d->gamma;
     g.tanh_g = d->tanh_g;
+    g.rLz_f = (float)(1.0 / d->Lz);
+    g.rgamma_f = (float)(1.0 / d->gamma);
+    g.tanh_g_f = (float)d->tanh_g;
     const double eps_z = 1e-12 * d->Lz;  // semilag.py:245-247
     g.z_lo = zf[0] + eps_z;
     g.z_hi = zf[nz] - eps_z;

@@ -32,6 +32,7 @@ struct Grid {
     int top_neumann;  // top wall: 0 no-slip (odd ghost), 1 free-slip (even ghost)
     int xy_lagged;    // BDF2 predictor xy RHS: 0 2R^n - R^{n-1} ('extrap'), 1 R^n ('lagged')
     double dx, dy, Lz, gamma, tanh_g, z_lo, z_hi, nu;
+    float rLz_f, rgamma_f, tanh_g_f;  // 1/Lz, 1/gamma, tanh(gamma) in float (z estimates)
     const double* z_c;      // nz+2 centres incl. ghost centres
     const double* z_f;      // nz+1 faces
     const double* dinv_c2;  // order-2 inverse Lagrange denominators, centre nodes
@@ -93,14 +94,21 @@ __device__ inline int pymod(long a, int n) {
     return (int)(r < 0 ? r + n : r);
 }
 
-// Python-style i mod n for 32-bit i.  Feet are CFL-bounded, so i is almost
-// always within one period of [0, n): two compares instead of a division.
-__device__ inline int wrap_idx(int i, int n) {
-    if ((unsigned)i < (unsigned)n) return i;
-    if (i < 0 && i >= -n) return i + n;
-    if (i >= n && i < 2 * n) return i - n;
+// Python-style i mod n for 32-bit i when i lies more than one period outside
+// [0, n) (feet displaced by more than the domain; non-finite feet).
+__device__ inline int wrap_far(int i, int n) {
     int r = i % n;
     return r < 0 ? r + n : r;
+}
+
+// Python-style i mod n for 32-bit i.  Feet are CFL-bounded, so i is almost
+// always within one period of [0, n): two selects (no divergent branch), and
+// the division only for a lane outside [-n, 2n).
+__device__ inline int wrap_idx(int i, int n) {
+    int r = i < 0 ? i + n : i;
+    r = r >= n ? r - n : r;
+    if (__builtin_expect((unsigned)r >= (unsigned)n, 0)) r = wrap_far(i, n);
+    return r;
 }
 
 // An integral-valued double (a floor) as an int.  The conversion instruction
@@ -142,20 +150,14 @@ __device__ inline int face_cell(const Grid& g, double z) {
 __device__ inline int face_cell_fast(const Grid& g, double z) {
     const int nz = g.nz;
     // the estimate is float throughout (the refinement below makes it exact)
-    const float zr = (float)z * (1.0f / (float)g.Lz), tg = (float)g.tanh_g;
-    const float rg = 1.0f / (float)g.gamma;
-    float kfe;
-    if (g.symmetric) {
-        float a = (2.0f * zr - 1.0f) * tg;
-        a = fminf(fmaxf(a, -0.9999999f), 0.9999999f);
-        float at = 0.5f * __logf((1.0f + a) / (1.0f - a));
-        kfe = (at * rg + 1.0f) * (0.5f * (float)nz);
-    } else {
-        float a = (1.0f - zr) * tg;
-        a = fminf(fmaxf(a, -0.9999999f), 0.9999999f);
-        float at = 0.5f * __logf((1.0f + a) / (1.0f - a));
-        kfe = (1.0f - at * rg) * (float)nz;
-    }
+    // (approximate reciprocals throughout: the estimate only has to land
+    // within a few cells of the interval)
+    const float zr = (float)z * g.rLz_f, tg = g.tanh_g_f, rg = g.rgamma_f;
+    const float a0 = g.symmetric ? (2.0f * zr - 1.0f) * tg : (1.0f - zr) * tg;
+    const float a = fminf(fmaxf(a0, -0.9999999f), 0.9999999f);
+    const float at = 0.5f * __logf((1.0f + a) * __builtin_amdgcn_rcpf(1.0f - a));
+    const float kfe = g.symmetric ? (at * rg + 1.0f) * (0.5f * (float)nz)
+                                  : (1.0f - at * rg) * (float)nz;
     int c = (int)floorf(fminf(fmaxf(kfe, 0.0f), (float)(nz - 1)));
     const double* zf = g.z_f;
     // both faces of the estimate at once (c + 1 <= nz is inside z_f); the

@@ -61,17 +61,19 @@ struct SBox {
 // Wavefront min / max with DPP lane moves (registers only): swaps within
 // quads, half-row and row mirrors give every lane its 16-lane row's value;
 // row broadcasts of lanes 15 and 31 fold the rows into lane 63, read back as
-// a uniform value.  A lane with no DPP source keeps its own value, which min
-// and max ignore.
+// a uniform value.  A lane with no DPP source receives the operation's
+// identity (INT_MAX for min, INT_MIN for max), which lets the compiler fold
+// each lane move into the min / max itself (one DPP-modified instruction).
 template <bool MIN>
 __device__ inline int s_wave_reduce(int v) {
-    auto op = [](int a, int b) { return MIN ? (b < a ? b : a) : (b > a ? b : a); };
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));  // row_mirror
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));  // row_bcast:15
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));  // row_bcast:31
+    constexpr int id = MIN ? INT_MAX : INT_MIN;
+    auto op = [](int a, int b) { return MIN ? min(a, b) : max(a, b); };
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x140, 0xF, 0xF, false));  // row_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x142, 0xA, 0xF, false));  // row_bcast:15
+    v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x143, 0xC, 0xF, false));  // row_bcast:31
     return __builtin_amdgcn_readlane(v, 63);
 }
 
