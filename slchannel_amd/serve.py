@@ -1,11 +1,17 @@
 """Serving loop: continuous batching of independent channel runs on one GPU.
 
 Requests (a channel config, a seed, a number of steps, an arrival time)
-arrive over time.  The loop admits arrived requests in FIFO order while the
-ensemble's HBM footprint plus the newcomer's (slc_ens_member_bytes) stays
-within the budget, initialises them on the device, and advances every
-admitted run by one SL-BDF2 step per iteration with the batched ensemble
-(one launch per substep for all runs, include/slc_ens.h).  Runs join and
+arrive over time.  The loop initialises admitted requests on the device and
+advances every admitted run by one SL-BDF2 step per iteration with the
+batched ensemble (one launch per substep for all runs, include/slc_ens.h).
+Admission needs the ensemble's HBM footprint plus the newcomer's
+(slc_ens_member_bytes) to stay within the budget, and follows the device's
+compute: the oldest waiting request joins when the batch is empty or its
+grid points fit the batch budget (`batch_points`), so runs that each fill
+the GPU take it one after the other at full speed, in arrival order; and any
+arrived small request (at most `small_points` grid points) joins at once
+while the small runs' points fit the budget -- it rides in the launches of
+whatever runs, costing its own arithmetic and no launches of its own.  Runs join and
 leave between steps: a run admitted while others are mid-flight bootstraps
 with BDF1 inside the same launches in which the others take BDF2 steps, and
 a run that has done its steps is copied out and its memory released, so a
@@ -163,6 +169,11 @@ def init_cache_bytes(device):
     return total
 
 
+def _points(config):
+    c = _config_grid(config)
+    return c["nx"] * c["ny"] * c["nz"]
+
+
 def _dt_of(req, dti):
     """cfl_dt's rule: dt = CFL / dti (dti > 1e-10, else CFL)."""
     return req.cfl / (dti if dti > 1e-10 else 1.0)
@@ -180,7 +191,8 @@ class Server:
     """Continuous-batching loop over one Ensemble."""
 
     def __init__(self, hbm_budget=None, max_members=512, order=6, storage=64, graph=False,
-                 keep_fields=False, device=None, guard_every=0, blowup_factor=2.0):
+                 keep_fields=False, device=None, guard_every=0, blowup_factor=2.0,
+                 batch_points=1 << 22, small_points=None):
         self.ens = Ensemble(order=order, storage=storage, device=device, graph=graph)
         self.device = self.ens.device
         if hbm_budget is None:
@@ -192,6 +204,10 @@ class Server:
         self.storage = storage
         self.guard_every = int(guard_every)
         self.blowup_factor = float(blowup_factor)
+        # compute-aware admission: grid points of the batch beyond which one
+        # step already fills the device, and the size of a rider
+        self.batch_points = int(batch_points)
+        self.small_points = int(small_points if small_points is not None else batch_points // 2)
         # admissions (initial fields, dt) run on one stream per grid size, so
         # they overlap the batched steps instead of queueing behind them, and
         # two admissions of one grid (which share its initialiser's plan)
@@ -242,6 +258,26 @@ class Server:
             self.ens.trim_pool(pooled - (need - room))
         return True
 
+    def _pick(self, pending, now, active_pts, small_pts, reserved, preparing):
+        """The next request to admit (its index in pending) or None: the
+        oldest arrived one when the batch is empty or it fits the batch
+        budget, else the oldest arrived small one that fits the small runs'
+        budget.  Both also need the HBM budget."""
+        if not pending or pending[0].arrival > now:
+            return None
+        head = _points(pending[0].config)
+        if (active_pts == 0 or active_pts + head <= self.batch_points) and \
+                self.fits(pending[0], reserved, preparing, trim=True):
+            return 0
+        for i, r in enumerate(pending[1:], 1):
+            if r.arrival > now:
+                break
+            p = _points(r.config)
+            if p <= self.small_points and small_pts + p <= self.batch_points and \
+                    self.fits(r, reserved, preparing, trim=True):
+                return i
+        return None
+
     def run(self, requests, log=None, inflight=3):
         """Serve `requests` (arrival times relative to the start); returns
         (results by rid, metrics dict).  Steps are pipelined: the host keeps
@@ -272,6 +308,7 @@ class Server:
         host = {"admit_s": 0.0, "step_s": 0.0, "retire_s": 0.0, "wait_s": 0.0}
 
         step_ms = []
+        pts = {}  # member id -> grid points
 
         def drain(keep):
             while len(queued) > keep:
@@ -289,9 +326,13 @@ class Server:
             # state is launched on the grid's stream and read back (dt, the
             # finiteness check) through pinned memory once that stream is done
             reserved = sum(x[6] for x in preparing)
-            if pending and pending[0].arrival <= now and \
-                    self.fits(pending[0], reserved, len(preparing), trim=True):
-                req = pending.pop(0)
+            active_pts = sum(pts.values()) + sum(_points(x[1].config) for x in preparing)
+            small_pts = sum(p for p in pts.values() if p <= self.small_points) + sum(
+                _points(x[1].config) for x in preparing
+                if _points(x[1].config) <= self.small_points)
+            pick = self._pick(pending, now, active_pts, small_pts, reserved, len(preparing))
+            if pick is not None:
+                req = pending.pop(pick)
                 st = self._stream_for(req.config)
                 with torch.cuda.stream(st):
                     fields, dt, probe = _prepare_async(req, self.device)
@@ -318,6 +359,7 @@ class Server:
                 res.arrival = req.arrival
                 guards[mid] = BlowupGuard(req.u_tau_nominal, self.blowup_factor)
                 active[mid] = (req, res)
+                pts[mid] = _points(req.config)
                 first_admit = res.admitted if first_admit is None else first_admit
                 if log:
                     c = _config_grid(req.config)
@@ -378,6 +420,7 @@ class Server:
             for mid, status in done:
                 req, res = active.pop(mid)
                 guards.pop(mid, None)
+                pts.pop(mid, None)
                 res.status = status
                 # copies are stream-ordered behind the member's last step;
                 # the arena is reused only by work queued after them

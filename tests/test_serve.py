@@ -1,11 +1,13 @@
-"""Serving loop (slchannel_amd/serve.py): continuous batching by HBM footprint.
+"""Serving loop (slchannel_amd/serve.py): continuous batching by HBM footprint
+and device compute.
 
 Requests of different grid sizes and step counts are served by one batched
 ensemble under an HBM budget that admits only part of them at once, so runs
 join while others are mid-flight (their BDF1 bootstrap inside the others'
-BDF2 launches) and leave when done.  Each result must equal the same request
-run alone on the plain per-run path (SLChannelFlow.step_sl_bdf2) bit for
-bit: fields, pressure, bulk velocity, forcing and clamp count.
+BDF2 launches) and leave when done.  Runs that each fill the device go one
+after the other; small ones ride along.  Each result must equal the same
+request run alone on the plain per-run path (SLChannelFlow.step_sl_bdf2) bit
+for bit: fields, pressure, bulk velocity, forcing and clamp count.
 """
 
 import math
@@ -129,3 +131,33 @@ def test_invalid_requests_rejected_alone(check):
         r = results[rid]
         check(f"request {rid} rejected_invalid", r.status == "rejected_invalid" and r.steps == 0,
               f"{r.status} {r.diag}")
+
+
+def test_compute_aware_admission(check):
+    """Runs that fill the device go one after the other in arrival order; a
+    small request that arrives behind a waiting large one rides along with
+    the running large one instead of queueing.  Every result equals its run
+    alone."""
+    from slchannel_amd.serve import Server, run_alone
+
+    big_a = _req(0, 48, 32, 32, 2800.0, 12, 0.0)
+    big_b = _req(1, 48, 32, 32, 3200.0, 3, 0.001)
+    small = _req(2, 24, 16, 24, 3000.0, 3, 0.002)
+    # 48x32x32 = 49152 grid points: two of them exceed the batch budget; the
+    # 9216-point request is a rider
+    srv = Server(keep_fields=True, batch_points=60000, small_points=20000)
+    srv.warmup([r.config for r in (big_a, small)])
+    log = []
+    results, _ = srv.run([big_a, big_b, small], log=log.append)
+    order = [int(m.split()[1]) for m in log]
+    active_at = {int(m.split()[1]): int(m.split("active=")[1]) for m in log}
+    check("all served", all(results[i].status == "ok" for i in range(3)),
+          f"{[results[i].status for i in range(3)]}")
+    check("admitted in the order large, small, large", order == [0, 2, 1], f"{log}")
+    # when the second large run is admitted the first has left: at most the
+    # rider and the newcomer are active
+    check("second large run waits for the first", active_at[1] <= 2, f"{log}")
+    for r in (big_a, big_b, small):
+        alone, _ = run_alone(r)
+        same = all(torch.equal(results[r.rid].fields[k], alone.fields[k]) for k in "uvwp")
+        check(f"request {r.rid}: == alone", same, "")
