@@ -6,6 +6,7 @@
 // path and the CPU oracle see the same numbers.
 #include <rocfft/rocfft.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -202,6 +203,7 @@ int slc::plan_create(const slc_grid_desc* d, int device, bool spectral, slc_plan
     g.rLz_f = (float)(1.0 / d->Lz);
     g.rgamma_f = (float)(1.0 / d->gamma);
     g.tanh_g_f = (float)d->tanh_g;
+    g.split_f = (float)(g.symmetric ? -d->gamma / (2.0 * nz) : d->gamma / (4.0 * nz));
     const double eps_z = 1e-12 * d->Lz;  // semilag.py:245-247
     g.z_lo = zf[0] + eps_z;
     g.z_hi = zf[nz] - eps_z;
@@ -289,6 +291,15 @@ int slc::plan_create(const slc_grid_desc* d, int device, bool spectral, slc_plan
             return rc;
         }
         p->nfoot = (double*)ptr;
+        {
+            long tiles = 0;
+            for (int c = 0; c < 3; ++c) tiles = std::max(tiles, gather_tiles(g, c));
+            if ((rc = dev_alloc(p, sizeof(unsigned) * (size_t)(tiles + 1), &ptr))) {
+                slc_plan_destroy(p);
+                return rc;
+            }
+            p->defer = (unsigned*)ptr;
+        }
         for (int c = 0; c < 3; ++c) {
             const int nzi = c == 2 ? nz - 1 : nz;
             void *a = nullptr, *b = nullptr, *q = nullptr;
@@ -462,7 +473,8 @@ int slc_gather(slc_plan* p, int comp, const double* field, int layout, const dou
                              0.0, out,
                              const_cast<double*>(xd), const_cast<double*>(yd),
                              const_cast<double*>(zd),
-                             reinterpret_cast<unsigned long long*>(nclamp), S(stream)));
+                             reinterpret_cast<unsigned long long*>(nclamp), S(stream),
+                             p->defer));
     } else if (layout == 0 && p->gather_shape > 0)
         HIP_OK(run_gather_tiled(p->g, comp, p->gather_shape, field, xd, yd, zd, out,
                                 reinterpret_cast<unsigned long long*>(nclamp), S(stream)));

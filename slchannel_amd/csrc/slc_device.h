@@ -33,6 +33,7 @@ struct Grid {
     int xy_lagged;    // BDF2 predictor xy RHS: 0 2R^n - R^{n-1} ('extrap'), 1 R^n ('lagged')
     double dx, dy, Lz, gamma, tanh_g, z_lo, z_hi, nu;
     float rLz_f, rgamma_f, tanh_g_f;  // 1/Lz, 1/gamma, tanh(gamma) in float (z estimates)
+    float split_f;  // slope of the centre-split estimate: -gamma/(2 nz) symmetric, gamma/(4 nz) bottom
     const double* z_c;      // nz+2 centres incl. ghost centres
     const double* z_f;      // nz+1 faces
     const double* dinv_c2;  // order-2 inverse Lagrange denominators, centre nodes
@@ -139,6 +140,45 @@ __device__ inline int face_cell(const Grid& g, double z) {
     return c;
 }
 
+// The float estimate of the face interval of z that face_cell_fast starts
+// from, clamped to [0, nz-1]; kfe receives the estimated face coordinate and
+// a the map's tanh argument at z.
+__device__ inline int face_estimate(const Grid& g, double z, float& kfe, float& a) {
+    const int nz = g.nz;
+    // float throughout, with approximate reciprocals: the estimate only has
+    // to land within a few cells of the interval (the callers make it exact)
+    const float zr = (float)z * g.rLz_f, tg = g.tanh_g_f, rg = g.rgamma_f;
+    const float a0 = g.symmetric ? (2.0f * zr - 1.0f) * tg : (1.0f - zr) * tg;
+    a = fminf(fmaxf(a0, -0.9999999f), 0.9999999f);
+    const float at = 0.5f * __logf((1.0f + a) * __builtin_amdgcn_rcpf(1.0f - a));
+    kfe = g.symmetric ? (at * rg + 1.0f) * (0.5f * (float)nz) : (1.0f - at * rg) * (float)nz;
+    return (int)floorf(fminf(fmaxf(kfe, 0.0f), (float)(nz - 1)));
+}
+
+__device__ inline int face_estimate(const Grid& g, double z) {
+    float kfe, a;
+    return face_estimate(g, z, kfe, a);
+}
+
+// Predicted centre split of face interval c for a z with estimate (kfe, a):
+// whether z >= z_c[c+1].  The centre is the faces' midpoint, which sits at
+// 0.5 + (z''/z') h / 8 of the interval in the map's coordinate (h its face
+// spacing) to first order; the slope is taken at z (a = tanh argument), so
+// the prediction is off only within ~1e-4 of a cell of the centre.
+__device__ inline bool split_predict(const Grid& g, int c, float kfe, float a) {
+    return kfe - (float)c >= 0.5f + g.split_f * a;
+}
+
+// Whether face_cell_fast returns the estimate c as it is: z inside its faces
+// lo = z_f[c], hi = z_f[c+1] (open at the clamped ends) and farther than
+// 1e-10 of the cell height from both.
+__device__ inline bool face_estimate_holds(const Grid& g, int c, double z, double lo,
+                                           double hi) {
+    const double band = 1e-10 * (hi - lo);
+    const bool bracketed = (c == 0 || z >= lo) && (c == g.nz - 1 || z < hi);
+    return bracketed && z - lo > band && hi - z > band;
+}
+
 // face_cell without the double-precision atanh.  A float32 estimate of the
 // inverse map is refined against the face table to the interval
 // z_f[c] <= z < z_f[c+1] (clamped to [0, nz-1]).  That interval is
@@ -146,35 +186,44 @@ __device__ inline int face_cell(const Grid& g, double z) {
 // happen for z within a few 1e-13 cell heights of a face (the analytic
 // inverse of a face table entry is k to ~1e-13); within 1e-10 of a cell height
 // of a face the analytic path itself runs, so the result always equals
-// face_cell(g, z).  z must lie in [z_lo, z_hi].
-__device__ inline int face_cell_fast(const Grid& g, double z) {
+// face_cell(g, z).  z must lie in [z_lo, z_hi].  split receives the centre
+// node z_c[c+1] of the interval, loaded in the same round as its faces (for
+// the estimated interval; again in the rare case the estimate moves).
+__device__ inline int face_cell_fast(const Grid& g, double z, double& split) {
     const int nz = g.nz;
-    // the estimate is float throughout (the refinement below makes it exact)
-    // (approximate reciprocals throughout: the estimate only has to land
-    // within a few cells of the interval)
-    const float zr = (float)z * g.rLz_f, tg = g.tanh_g_f, rg = g.rgamma_f;
-    const float a0 = g.symmetric ? (2.0f * zr - 1.0f) * tg : (1.0f - zr) * tg;
-    const float a = fminf(fmaxf(a0, -0.9999999f), 0.9999999f);
-    const float at = 0.5f * __logf((1.0f + a) * __builtin_amdgcn_rcpf(1.0f - a));
-    const float kfe = g.symmetric ? (at * rg + 1.0f) * (0.5f * (float)nz)
-                                  : (1.0f - at * rg) * (float)nz;
-    int c = (int)floorf(fminf(fmaxf(kfe, 0.0f), (float)(nz - 1)));
+    int c = face_estimate(g, z);
     const double* zf = g.z_f;
     // both faces of the estimate at once (c + 1 <= nz is inside z_f); the
     // refinement loops only run when the estimate does not bracket z, which
     // is exactly when they would take a step
     double lo = zf[c], hi = zf[c + 1];
+    split = g.z_c[c + 1];
     if (!((c == 0 || z >= lo) && (c == nz - 1 || z < hi))) {
         int guard = 0;
         while (c > 0 && z < zf[c] && guard < 8) { --c; ++guard; }
         while (c < nz - 1 && z >= zf[c + 1] && guard < 16) { ++c; ++guard; }
         lo = zf[c];
         hi = zf[c + 1];
+        split = g.z_c[c + 1];
     }
-    const double band = 1e-10 * (hi - lo);
-    const bool bracketed = (c == 0 || z >= lo) && (c == nz - 1 || z < hi);
-    if (!bracketed || z - lo <= band || hi - z <= band) return face_cell(g, z);
+    if (!face_estimate_holds(g, c, z, lo, hi)) {
+        c = face_cell(g, z);
+        split = g.z_c[c + 1];
+    }
     return c;
+}
+
+__device__ inline int face_cell_fast(const Grid& g, double z) {
+    double split;
+    return face_cell_fast(g, z, split);
+}
+
+// Interval index m0 of component C's z nodes containing z
+// (semilag.py:413-420), from the face interval c and its centre split.
+template <int C>
+__device__ inline int locate_split(int c, double split, double z) {
+    if (!Comp<C>::centers) return c;
+    return c + (z >= split ? 1 : 0);
 }
 
 // Interval index m0 of the component's z nodes containing z (semilag.py:413-420).
@@ -372,6 +421,62 @@ __device__ inline double trilinear(const Grid& g, const TV& V, double xdx, doubl
     return out;
 }
 
+// trilinear with the z node interval m0 given (the reference's locate
+// result; k0 = m0 at order 2 -- no clamp is needed: m0 <= NZ - 2 for a
+// clamped z).
+// x/y part of the trilinear sample of component C: order-2 uniform weights
+// ((t - 1) * (-1), t * 1) and the wrapped column indices.
+template <int C>
+__device__ inline void trilinear_xy(const Grid& g, double xdx, double ydy, double* wx,
+                                    double* wy, int* ix, int* iy) {
+    double sx = xdx + Comp<C>::shift_x;
+    double ix0 = floor(sx);
+    double tx = sx - ix0;
+    double sy = ydy + Comp<C>::shift_y;
+    double iy0 = floor(sy);
+    double ty = sy - iy0;
+    const int fx = floor_int(ix0), fy = floor_int(iy0);
+    ix[0] = wrap_idx(fx, g.nx);
+    ix[1] = wrap_idx(fx + 1, g.nx);
+    iy[0] = wrap_idx(fy, g.ny);
+    iy[1] = wrap_idx(fy + 1, g.ny);
+    wx[0] = (tx - 1.0) * (-1.0);
+    wx[1] = tx * 1.0;
+    wy[0] = (ty - 1.0) * (-1.0);
+    wy[1] = ty * 1.0;
+}
+
+template <int C, typename TV = TrajVel>
+__device__ inline double trilinear_m(const Grid& g, const TV& V, double xdx, double ydy,
+                                     double zc, int m0) {
+    double wx[2], wy[2], wz[2];
+    int ix[2], iy[2];
+    trilinear_xy<C>(g, xdx, ydy, wx, wy, ix, iy);
+    const int k0 = m0;
+    const double* zn = znodes<C>(g);
+    const double* dinv = Comp<C>::centers ? g.dinv_c2 : g.dinv_f2;
+    double d0 = zc - zn[k0], d1 = zc - zn[k0 + 1];
+    wz[0] = d1 * dinv[k0 * 2 + 0];
+    wz[1] = d0 * dinv[k0 * 2 + 1];
+    double out = 0.0;
+    bool first = true;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int base = node_addr<C, TV::packed>(g, ix[i], iy[j], k0);
+            double wij = wx[i] * wy[j];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                double term = (wij * wz[k]) * V.at(C, base + k);
+                out = first ? term : out + term;
+                first = false;
+            }
+        }
+    }
+    return out;
+}
+
 // Tricubic trajectory sample of component C (traj_interp_order = 4): the
 // order-4 stencil of semilag.py:471-500 (no clamp counting; zc already
 // clamped) and the sequential (i, j, k) sum of _gather_interp
@@ -522,7 +627,9 @@ __device__ inline void stencil_base(const Grid& g, double x, double y, double z,
     ty = sy - iy0;
     bx = floor_int(ix0) - h;
     by = floor_int(iy0) - h;
-    int m0 = locate_from_cell<C>(g, face_cell_fast(g, z), z);
+    double split;
+    const int c = face_cell_fast(g, z, split);
+    int m0 = locate_split<C>(c, split, z);
     int kb = m0 - h;
     const int NZ = Comp<C>::NZ(g);
     if (kb < 0) kb = 0;

@@ -147,24 +147,35 @@ __global__ __launch_bounds__(256) void zsolve_kernel(Grid g, double* __restrict_
     zsolve_body<C>(g, F, tab, beta, dz_c, dz_f, kZCols, blockIdx.x, lds, cbase);
 }
 
-template <int C, int CH>
+// columns per wavefront of the single-run z-solve (compile-time sweep knob)
+#ifndef SLC_ZSCAN_K
+#define SLC_ZSCAN_K 1
+#endif
+constexpr int kZScanK = SLC_ZSCAN_K;
+
+template <int C, int CH, int K>
 __global__ __launch_bounds__(256) void zsolve_scan_kernel(Grid g, double* __restrict__ F,
                                                           const double* __restrict__ tab,
                                                           double beta,
                                                           const double* __restrict__ dz_c,
                                                           const double* __restrict__ dz_f) {
-    const long col = (long)blockIdx.x * kZScanCols + (threadIdx.x >> 6);
-    if (col < (long)g.nx * g.ny) zsolve_scan_col<C, CH>(g, F, tab, col, beta, dz_c, dz_f);
+    const long ncols = (long)g.nx * g.ny;
+    const long col0 = ((long)blockIdx.x * kZScanCols + (threadIdx.x >> 6)) * K;
+    if (col0 < ncols) zsolve_scan_cols<C, CH, K>(g, F, tab, col0, ncols, beta, dz_c, dz_f);
 }
 
 template <int C>
 void launch_zsolve_scan(const Grid& g, double* f, const double* tab, int ch, double beta,
                         const double* dz_c, const double* dz_f, hipStream_t s) {
     const long ncols = (long)g.nx * g.ny;
-    const dim3 grid((unsigned)((ncols + kZScanCols - 1) / kZScanCols)), blk(256);
-#define SLC_ZSCAN(CHV)                                                                      \
-    hipLaunchKernelGGL((zsolve_scan_kernel<C, CHV>), grid, blk, 0, s, g, f, tab, beta, dz_c, \
-                       dz_f)
+    // 16 unknowns per lane: one column per wavefront (two would not fit the
+    // registers)
+    const long K = ch <= 8 ? kZScanK : 1;
+    const long per = (long)kZScanCols * K;
+    const dim3 grid((unsigned)((ncols + per - 1) / per)), blk(256);
+#define SLC_ZSCAN(CHV)                                                                        \
+    hipLaunchKernelGGL((zsolve_scan_kernel<C, CHV, (CHV <= 8 ? kZScanK : 1)>), grid, blk, 0, s, \
+                       g, f, tab, beta, dz_c, dz_f)
     switch (ch) {
         case 1: SLC_ZSCAN(1); break;
         case 2: SLC_ZSCAN(2); break;

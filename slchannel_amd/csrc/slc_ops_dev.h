@@ -300,88 +300,135 @@ __device__ inline void zsolve_explicit(const Grid& g, double* p, int j0, int n, 
     }
 }
 
+// K columns col0 .. col0 + K - 1 (those below ncols) per wavefront: the
+// loads of all K are issued before any is solved (more bytes in flight per
+// wavefront; the factor tables are shared), then each column is solved
+// exactly as a column alone.
+template <int C, int CH, int K>
+__device__ inline void zsolve_scan_cols(const Grid& g, double* __restrict__ F,
+                                        const double* __restrict__ tab, long col0, long ncols,
+                                        double beta = 0.0,
+                                        const double* __restrict__ dz_c = nullptr,
+                                        const double* __restrict__ dz_f = nullptr) {
+    const int lane = threadIdx.x & 63;
+    const int n = (C == 2) ? g.nz - 1 : g.nz;
+    const double* A = tab;
+    const double* CP = tab + n;
+    const double* INV = tab + 2 * n;
+    const int j0 = lane * CH;
+    double* Fc[K];
+    bool ok[K];
+    double p[K][CH], q[K][CH];
+#pragma unroll
+    for (int kc = 0; kc < K; ++kc) {
+        const long col = col0 + kc;
+        ok[kc] = col < ncols;
+        const unsigned ug = (unsigned)(ok[kc] ? col : col0);
+        const int I = 1 + (int)(ug / (unsigned)g.ny), J = 1 + (int)(ug % (unsigned)g.ny);
+        Fc[kc] = F + ((long)I * Comp<C>::SY(g) + J) * Comp<C>::SZ(g) + 1;
+#pragma unroll
+        for (int i = 0; i < CH; ++i) p[kc][i] = (ok[kc] && j0 + i <= n) ? Fc[kc][j0 + i] : 0.0;
+    }
+    if (beta != 0.0) {
+#pragma unroll
+        for (int kc = 0; kc < K; ++kc)
+            zsolve_explicit<C, CH>(g, p[kc], j0, n, beta, dz_c, dz_f, ok[kc] ? Fc[kc][-1] : 0.0);
+    }
+    // forward: dp_j = p_j + q_j y with y = dp_{j0-1}
+    double P[K], Q[K];
+#pragma unroll
+    for (int kc = 0; kc < K; ++kc) {
+        double pr = 0.0, qr = 1.0;
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+            const int j = j0 + i;
+            if (j < n) {
+                if (j == 0) {
+                    pr = p[kc][i] * INV[0];
+                    qr = 0.0;
+                } else {
+                    const double a = A[j], r = INV[j];
+                    pr = (p[kc][i] - a * pr) * r;
+                    qr = -(a * qr) * r;
+                }
+            }
+            p[kc][i] = pr;
+            q[kc][i] = qr;
+        }
+        P[kc] = pr;
+        Q[kc] = qr;
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+        for (int kc = 0; kc < K; ++kc) {
+            const double Pp = __shfl_up(P[kc], off), Qp = __shfl_up(Q[kc], off);
+            if (lane >= off) {
+                P[kc] = P[kc] + Q[kc] * Pp;
+                Q[kc] = Q[kc] * Qp;
+            }
+        }
+    }
+#pragma unroll
+    for (int kc = 0; kc < K; ++kc) {
+        double y = __shfl_up(P[kc], 1);
+        if (lane == 0) y = 0.0;
+#pragma unroll
+        for (int i = 0; i < CH; ++i) p[kc][i] = p[kc][i] + q[kc][i] * y;
+    }
+    // backward: x_j = r_j + t_j z with z = x_{j0+CH}
+    double R[K], T[K];
+#pragma unroll
+    for (int kc = 0; kc < K; ++kc) {
+        double rr = 0.0, tr = 1.0;
+#pragma unroll
+        for (int i = CH - 1; i >= 0; --i) {
+            const int j = j0 + i;
+            if (j < n) {
+                if (j == n - 1) {
+                    rr = p[kc][i];
+                    tr = 0.0;
+                } else {
+                    const double c = CP[j];
+                    rr = p[kc][i] - c * rr;
+                    tr = -(c * tr);
+                }
+            }
+            p[kc][i] = rr;
+            q[kc][i] = tr;
+        }
+        R[kc] = rr;
+        T[kc] = tr;
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+        for (int kc = 0; kc < K; ++kc) {
+            const double Rn = __shfl_down(R[kc], off), Tn = __shfl_down(T[kc], off);
+            if (lane + off < 64) {
+                R[kc] = R[kc] + T[kc] * Rn;
+                T[kc] = T[kc] * Tn;
+            }
+        }
+    }
+#pragma unroll
+    for (int kc = 0; kc < K; ++kc) {
+        double z = __shfl_down(R[kc], 1);
+        if (lane == 63) z = 0.0;
+        if (!ok[kc]) continue;
+#pragma unroll
+        for (int i = 0; i < CH; ++i)
+            if (j0 + i < n) Fc[kc][j0 + i] = p[kc][i] + q[kc][i] * z;
+    }
+}
+
 template <int C, int CH>
 __device__ inline void zsolve_scan_col(const Grid& g, double* __restrict__ F,
                                        const double* __restrict__ tab, long col,
                                        double beta = 0.0,
                                        const double* __restrict__ dz_c = nullptr,
                                        const double* __restrict__ dz_f = nullptr) {
-    const int lane = threadIdx.x & 63;
-    const int n = (C == 2) ? g.nz - 1 : g.nz;
-    const unsigned ug = (unsigned)col;
-    const int I = 1 + (int)(ug / (unsigned)g.ny), J = 1 + (int)(ug % (unsigned)g.ny);
-    double* Fc = F + ((long)I * Comp<C>::SY(g) + J) * Comp<C>::SZ(g) + 1;
-    const double* A = tab;
-    const double* CP = tab + n;
-    const double* INV = tab + 2 * n;
-    const int j0 = lane * CH;
-    double p[CH], q[CH];
-#pragma unroll
-    for (int i = 0; i < CH; ++i) p[i] = (j0 + i <= n) ? Fc[j0 + i] : 0.0;
-    if (beta != 0.0) zsolve_explicit<C, CH>(g, p, j0, n, beta, dz_c, dz_f, Fc[-1]);
-    // forward: dp_j = p_j + q_j y with y = dp_{j0-1}
-    double pr = 0.0, qr = 1.0;
-#pragma unroll
-    for (int i = 0; i < CH; ++i) {
-        const int j = j0 + i;
-        if (j < n) {
-            if (j == 0) {
-                pr = p[i] * INV[0];
-                qr = 0.0;
-            } else {
-                const double a = A[j], r = INV[j];
-                pr = (p[i] - a * pr) * r;
-                qr = -(a * qr) * r;
-            }
-        }
-        p[i] = pr;
-        q[i] = qr;
-    }
-    double P = pr, Q = qr;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const double Pp = __shfl_up(P, off), Qp = __shfl_up(Q, off);
-        if (lane >= off) {
-            P = P + Q * Pp;
-            Q = Q * Qp;
-        }
-    }
-    double y = __shfl_up(P, 1);
-    if (lane == 0) y = 0.0;
-#pragma unroll
-    for (int i = 0; i < CH; ++i) p[i] = p[i] + q[i] * y;
-    // backward: x_j = r_j + t_j z with z = x_{j0+CH}
-    double rr = 0.0, tr = 1.0;
-#pragma unroll
-    for (int i = CH - 1; i >= 0; --i) {
-        const int j = j0 + i;
-        if (j < n) {
-            if (j == n - 1) {
-                rr = p[i];
-                tr = 0.0;
-            } else {
-                const double c = CP[j];
-                rr = p[i] - c * rr;
-                tr = -(c * tr);
-            }
-        }
-        p[i] = rr;
-        q[i] = tr;
-    }
-    double R = rr, T = tr;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const double Rn = __shfl_down(R, off), Tn = __shfl_down(T, off);
-        if (lane + off < 64) {
-            R = R + T * Rn;
-            T = T * Tn;
-        }
-    }
-    double z = __shfl_down(R, 1);
-    if (lane == 63) z = 0.0;
-#pragma unroll
-    for (int i = 0; i < CH; ++i)
-        if (j0 + i < n) Fc[j0 + i] = p[i] + q[i] * z;
+    zsolve_scan_cols<C, CH, 1>(g, F, tab, col, col + 1, beta, dz_c, dz_f);
 }
 
 // Runtime CH dispatch of zsolve_scan_col (ch from zsolve_scan_ch), theta = 1.

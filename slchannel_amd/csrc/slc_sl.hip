@@ -142,7 +142,9 @@ __global__ void locate_kernel(Grid g, const double* __restrict__ z, long n, int*
     if (i >= n) return;
     const double zc = clampd(z[i], g.z_lo, g.z_hi);
     exact[i] = locate_from_cell<C>(g, face_cell(g, zc), zc);
-    fast[i] = locate_from_cell<C>(g, face_cell_fast(g, zc), zc);
+    double split;
+    const int c = face_cell_fast(g, zc, split);
+    fast[i] = locate_split<C>(c, split, zc);
 }
 
 hipError_t run_locate(const Grid& g, int comp, const double* z, long n, int* exact, int* fast,

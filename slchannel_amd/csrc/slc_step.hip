@@ -39,19 +39,45 @@ template <typename T, int C, int P, int MODE, int TI, int TJ, int TK, int LDS_BY
 __global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(WAVES))) void sl_tile_kernel(
     Grid g, ArrivalTabs A, TrajVel V, const double* __restrict__ F,
     const double* __restrict__ Fm, const T* __restrict__ Sn, const T* __restrict__ Sm,
-    double dt, double* __restrict__ out, Feet ft, unsigned long long* nclamp) {
+    double dt, double* __restrict__ out, Feet ft, unsigned long long* nclamp,
+    unsigned* defer) {
     __shared__ T lds[LDS_BYTES / sizeof(T)];
     __shared__ int sh[MODE == MODE_BDF2 ? 12 : 6];
     __shared__ int s_cnt;
     __shared__ FootSt nst[MODE == MODE_BDF2 ? TI * TJ * TK : 1];  // near stencils (MODE_BDF2)
     sl_tile_body<T, C, P, MODE, TI, TJ, TK, LDS_BYTES>(g, A, V, F, Fm, Sn, Sm, dt, out, ft,
                                                        nclamp, xcd_block(gridDim.x), lds, sh,
-                                                       &s_cnt, nst);
+                                                       &s_cnt, nst, defer);
 }
 
+// The gather at given feet for the tiles a first pass deferred (their boxes
+// exceed its LDS budget): the tile indices are read from list[1..list[0]];
+// blocks beyond the count exit.  A tile that does not fit this budget
+// either takes the direct-load path.  gfx950 gives one workgroup up to
+// 160 KB of LDS; 76 KB keeps two of these blocks per CU and holds the box of
+// a 4x4x16 tile whose feet spread over +-5 cells in x and y at order 6
+// (at most 20 x 20 x 21 doubles and the row bases).
+constexpr int kWideLds = 77824;
+
+template <int C, int P>
+__global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(2))) void gather_wide_kernel(
+    Grid g, ArrivalTabs A, const double* __restrict__ F, Feet ft, double* __restrict__ out,
+    unsigned long long* nclamp, const unsigned* __restrict__ list) {
+    __shared__ double lds[kWideLds / sizeof(double)];
+    __shared__ int sh[6];
+    __shared__ int s_cnt;
+    if (blockIdx.x >= list[0]) return;
+    const TrajVel V{{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    sl_tile_body<double, C, P, MODE_GATHER, SLC_GTILE_TI, SLC_GTILE_TJ, SLC_GTILE_TK, kWideLds>(
+        g, A, V, F, F, F, F, 0.0, out, ft, nclamp, (long)list[1 + blockIdx.x], lds, sh, &s_cnt);
+}
+
+__global__ void zero_u32_kernel(unsigned* p) { *p = 0u; }
+
 // Arrival tables of component C: x/dx, y/dy of the arrival columns and the
-// face cell of each arrival z, evaluated with the very functions the
-// trajectory sampling uses (so looked-up values are the computed ones).
+// face cell and centre split of each arrival z (2 c + split), evaluated with
+// the very functions the trajectory sampling uses (so looked-up values are
+// the computed ones).
 template <int C>
 __global__ void arrival_tabs_kernel(Grid g, double* xdx, double* ydy, int* cell) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -68,7 +94,9 @@ __global__ void arrival_tabs_kernel(Grid g, double* xdx, double* ydy, int* cell)
     }
     if (t < nzi) {
         arrival<C>(g, (long)t, xa, ya, za, I, J, K);
-        cell[t] = face_cell(g, clampd(za, g.z_lo, g.z_hi));
+        const double zc = clampd(za, g.z_lo, g.z_hi);
+        const int c = face_cell(g, zc);
+        cell[t] = 2 * c + (zc >= g.z_c[c + 1] ? 1 : 0);
     }
 }
 
@@ -170,6 +198,7 @@ struct StepArgs {
     double* out;
     Feet ft;
     unsigned long long* ncl;
+    unsigned* defer;  // MODE_GATHER: tile list of the wide second pass, or null
 };
 
 template <typename T, int C, int P, int MODE>
@@ -184,9 +213,16 @@ static void launch_mode(const Grid& g, const ArrivalTabs& A, const TrajVel& V,
     const unsigned tiles = (unsigned)(((long)(g.nx + TI - 1) / TI) *
                                       ((g.ny + TJ - 1) / TJ) * ((nzi + TK - 1) / TK));
     if (tiles == 0) return;
+    unsigned* defer = (MODE == MODE_GATHER) ? a.defer : nullptr;
+    if (defer) hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, defer);
     hipLaunchKernelGGL((sl_tile_kernel<T, C, P, MODE, TI, TJ, TK, LDS, W>), dim3(tiles),
                        dim3(kSB), 0, s, g, A, V, a.F, a.Fm, (const T*)a.Sn, (const T*)a.Sm, a.dt,
-                       a.out, a.ft, a.ncl);
+                       a.out, a.ft, a.ncl, defer);
+    if constexpr (MODE == MODE_GATHER && sizeof(T) == 8) {
+        if (defer)
+            hipLaunchKernelGGL((gather_wide_kernel<C, P>), dim3(tiles), dim3(kSB), 0, s, g, A,
+                               (const double*)a.Sn, a.ft, a.out, a.ncl, defer);
+    }
 }
 
 template <typename T, int C, int MODE>
@@ -222,16 +258,22 @@ static void launch_t(const Grid& g, int comp, int mode, const ArrivalTabs& A, co
 hipError_t run_tile_step(const Grid& g, int comp, int mode, const ArrivalTabs& A,
                          const TrajVel& V, const double* F, const double* Fm, const float* Fn32,
                          const float* Fm32, double dt, double* out, double* fx, double* fy,
-                         double* fz, unsigned long long* ncl, hipStream_t s) {
+                         double* fz, unsigned long long* ncl, hipStream_t s, unsigned* defer) {
     const Feet ft{fx, fy, fz};
     if (Fn32 || Fm32) {
-        StepArgs a{F, Fm, Fn32, Fm32, dt, out, ft, ncl};
+        StepArgs a{F, Fm, Fn32, Fm32, dt, out, ft, ncl, nullptr};
         launch_t<float>(g, comp, mode, A, V, a, s);
     } else {
-        StepArgs a{F, Fm, F, Fm, dt, out, ft, ncl};
+        StepArgs a{F, Fm, F, Fm, dt, out, ft, ncl, defer};
         launch_t<double>(g, comp, mode, A, V, a, s);
     }
     return hipGetLastError();
+}
+
+long gather_tiles(const Grid& g, int comp) {
+    const long nzi = (comp == 2) ? g.nz - 1 : g.nz;
+    return ((long)(g.nx + SLC_GTILE_TI - 1) / SLC_GTILE_TI) *
+           ((g.ny + SLC_GTILE_TJ - 1) / SLC_GTILE_TJ) * ((nzi + SLC_GTILE_TK - 1) / SLC_GTILE_TK);
 }
 
 }  // namespace slc

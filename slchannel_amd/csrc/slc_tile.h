@@ -81,24 +81,30 @@ __device__ inline int s_wave_min(int v) { return s_wave_reduce<true>(v); }
 
 __device__ inline int s_wave_max(int v) { return s_wave_reduce<false>(v); }
 
-// Workgroup union of the lanes' boxes, left in sh[0..5] (xmin, xmax, ymin,
-// ymax, kmin, kmax); ends with a barrier.
-__device__ inline void s_box_reduce(SBox l, int* sh) {
+// The workgroup unions of two boxes at once (one pair of barriers): b's left
+// in sh[0..5] and returned (wave-uniform), bn's left in sh[6..11].
+__device__ inline SBox s_block_box2(SBox l, SBox ln, int* sh) {
     if (threadIdx.x == 0) {
-        sh[0] = INT_MAX; sh[1] = INT_MIN;
-        sh[2] = INT_MAX; sh[3] = INT_MIN;
-        sh[4] = INT_MAX; sh[5] = INT_MIN;
+#pragma unroll
+        for (int q = 0; q < 12; q += 2) {
+            sh[q] = INT_MAX;
+            sh[q + 1] = INT_MIN;
+        }
     }
     __syncthreads();
-    int a = s_wave_min(l.xmin), b = s_wave_max(l.xmax);
-    int c = s_wave_min(l.ymin), d = s_wave_max(l.ymax);
-    int e = s_wave_min(l.kmin), f = s_wave_max(l.kmax);
+    int v[12] = {s_wave_min(l.xmin), s_wave_max(l.xmax), s_wave_min(l.ymin),
+                 s_wave_max(l.ymax), s_wave_min(l.kmin), s_wave_max(l.kmax),
+                 s_wave_min(ln.xmin), s_wave_max(ln.xmax), s_wave_min(ln.ymin),
+                 s_wave_max(ln.ymax), s_wave_min(ln.kmin), s_wave_max(ln.kmax)};
     if ((threadIdx.x & 63) == 0) {
-        atomicMin(&sh[0], a); atomicMax(&sh[1], b);
-        atomicMin(&sh[2], c); atomicMax(&sh[3], d);
-        atomicMin(&sh[4], e); atomicMax(&sh[5], f);
+#pragma unroll
+        for (int q = 0; q < 12; q += 2) {
+            atomicMin(&sh[q], v[q]);
+            atomicMax(&sh[q + 1], v[q + 1]);
+        }
     }
     __syncthreads();
+    return SBox{sh[0], sh[1], sh[2], sh[3], sh[4], sh[5]};
 }
 
 __device__ inline SBox s_block_box(SBox l, int* sh) {
@@ -124,25 +130,119 @@ __device__ inline SBox s_block_box(SBox l, int* sh) {
 
 // Trajectory samples of the three U* components at one point (they share
 // x/dx, y/dy and the face cell of z; semilag.py:512-528 recomputes them with
-// identical results).
-__device__ inline void sample3(const Grid& g, const TrajA& V, double x, double y, double z,
-                               double& us, double& vs, double& ws) {
-    const double zc = clampd(z, g.z_lo, g.z_hi);
-    const double xdx = x / g.dx, ydy = y / g.dy;
-    const int cell = face_cell_fast(g, zc);
-    us = trilinear<0>(g, V, xdx, ydy, zc, cell);
-    vs = trilinear<1>(g, V, xdx, ydy, zc, cell);
-    ws = trilinear<2>(g, V, xdx, ydy, zc, cell);
+// identical results).  Two rounds of loads: the faces of the estimated
+// interval, then the corners.
+// xdx = x/dx, ydy = y/dy and zc the clamped z.
+__device__ inline void sample3_exact(const Grid& g, const TrajA& V, double xdx, double ydy,
+                                     double zc, double& us, double& vs, double& ws) {
+    double split;
+    const int c = face_cell_fast(g, zc, split);
+    const int mc = locate_split<0>(c, split, zc);
+    us = trilinear_m<0>(g, V, xdx, ydy, zc, mc);
+    vs = trilinear_m<1>(g, V, xdx, ydy, zc, mc);
+    ws = trilinear_m<2>(g, V, xdx, ydy, zc, c);
 }
 
-// Arrival-point trajectory sample with precomputed x/dx, y/dy and face cell
-// (the same values sample3 computes; see ArrivalTabs).
-__device__ inline void sample3_at(const Grid& g, const TrajA& V, double xdx, double ydy,
-                                  double z, int cell, double& us, double& vs, double& ws) {
+// Column values of component C at levels k0, k0+1 of its four trilinear
+// columns, with the x/y weights of trilinear_m.
+template <int C>
+__device__ inline void trilinear_cols(const Grid& g, const TrajA& V, double xdx, double ydy,
+                                      int k0, double* wx, double* wy, double (&v)[2][2][2]) {
+    int ix[2], iy[2];
+    trilinear_xy<C>(g, xdx, ydy, wx, wy, ix, iy);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int base = node_addr<C, false>(g, ix[i], iy[j], k0);
+            v[i][j][0] = V.a[C][base];
+            v[i][j][1] = V.a[C][base + 1];
+        }
+}
+
+// trilinear_m's sum, over column values read by trilinear_cols.
+__device__ inline double trilinear_sum(const double* wx, const double* wy, const double* wz,
+                                       const double (&v)[2][2][2]) {
+    double out = 0.0;
+    bool first = true;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            double wij = wx[i] * wy[j];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                double term = (wij * wz[k]) * v[i][j][k];
+                out = first ? term : out + term;
+                first = false;
+            }
+        }
+    }
+    return out;
+}
+
+// sample3_exact in one round of loads, for a point traced from the arrival
+// (za, K) of component C.  The face interval c and the centred components'
+// node interval m = c + s (s: z >= z_c[c+1]) are predicted without loads, and
+// the faces, centre nodes, inverse denominators and the three components'
+// columns of that prediction are read at once (every index inside its table
+// or field for c in [0, nz-1], s in {0, 1}).  Near the arrival the
+// prediction is exact: for C = u, v, za = z_c[K] is the split of face
+// interval K-1; for C = w, za = z_f[K] is the face between intervals K-1 and
+// K.  Elsewhere c is the float estimate of face_cell_fast and s comes from
+// split_predict.  The loaded faces and centre then decide: where they confirm
+// the prediction and z is outside the faces' 1e-10 band, c is the interval
+// face_cell_fast returns (the one interval bracketing z) and m the node
+// interval of locate_split.  Otherwise (a few lanes in 1e5 on stretched
+// production grids) c and m are located as sample3_exact does and the values
+// read again.  Either way the sums are trilinear_m's operations on the same
+// values, so the result is sample3_exact's.
+template <int C>
+__device__ inline void sample3(const Grid& g, const TrajA& V, double x, double y, double z,
+                               double za, int K, double& us, double& vs, double& ws) {
     const double zc = clampd(z, g.z_lo, g.z_hi);
-    us = trilinear<0>(g, V, xdx, ydy, zc, cell);
-    vs = trilinear<1>(g, V, xdx, ydy, zc, cell);
-    ws = trilinear<2>(g, V, xdx, ydy, zc, cell);
+    const double xdx = x / g.dx, ydy = y / g.dy;
+    float kfe, a;
+    int c = face_estimate(g, zc, kfe, a);
+    if (!Comp<C>::centers && (c == K - 1 || c == K)) c = zc >= za ? K : K - 1;
+    bool s = split_predict(g, c, kfe, a);
+    if (Comp<C>::centers && c == K - 1) s = zc >= za;
+    int m = c + (s ? 1 : 0);
+    for (bool located = false;; located = true) {
+        const double lo = g.z_f[c], hi = g.z_f[c + 1];
+        const double n0 = g.z_c[m], n1 = g.z_c[m + 1];
+        const double p0 = g.dinv_c2[2 * m], p1 = g.dinv_c2[2 * m + 1];
+        const double f0 = g.dinv_f2[2 * c], f1 = g.dinv_f2[2 * c + 1];
+        double uwx[2], uwy[2], vwx[2], vwy[2], wwx[2], wwy[2];
+        double uv[2][2][2], vv[2][2][2], wv[2][2][2];
+        trilinear_cols<0>(g, V, xdx, ydy, m, uwx, uwy, uv);
+        trilinear_cols<1>(g, V, xdx, ydy, m, vwx, vwy, vv);
+        trilinear_cols<2>(g, V, xdx, ydy, c, wwx, wwy, wv);
+        const double split = m > c ? n0 : n1;  // z_c[c + 1]
+        if (located || (face_estimate_holds(g, c, zc, lo, hi) && (zc >= split) == (m > c))) {
+            const double wzc[2] = {(zc - n1) * p0, (zc - n0) * p1};
+            const double wzf[2] = {(zc - hi) * f0, (zc - lo) * f1};
+            us = trilinear_sum(uwx, uwy, wzc, uv);
+            vs = trilinear_sum(vwx, vwy, wzc, vv);
+            ws = trilinear_sum(wwx, wwy, wzf, wv);
+            return;
+        }
+        double sp;
+        c = face_cell_fast(g, zc, sp);
+        m = locate_split<0>(c, sp, zc);
+    }
+}
+
+// Arrival-point trajectory sample with precomputed x/dx, y/dy and z location
+// (csel = 2 * face cell + centre split: the values sample3_exact computes; see
+// ArrivalTabs).
+__device__ inline void sample3_at(const Grid& g, const TrajA& V, double xdx, double ydy,
+                                  double z, int csel, double& us, double& vs, double& ws) {
+    const double zc = clampd(z, g.z_lo, g.z_hi);
+    const int c = csel >> 1, mc = c + (csel & 1);
+    us = trilinear_m<0>(g, V, xdx, ydy, zc, mc);
+    vs = trilinear_m<1>(g, V, xdx, ydy, zc, mc);
+    ws = trilinear_m<2>(g, V, xdx, ydy, zc, c);
 }
 
 // Per-foot stencil data kept between the phases.
@@ -214,18 +314,31 @@ __device__ inline bool stage_box(const Grid& g, const T* __restrict__ S, const S
     return fits;
 }
 
-// Order-P gather of one foot: Lagrange weights, then the staged z -> y -> x
-// contraction with FMAs, from the LDS box when it was staged, else from S.
-template <typename T, int C, int P>
-__device__ inline double contract(const Grid& g, const FootSt& f, const SBox& b, bool fits,
-                                  const T* __restrict__ lds, int BY, int BZ,
-                                  const T* __restrict__ S) {
+// z weights of one foot's order-P stencil against the node tables
+// (semilag.py:441-454): the part of the weights that reads memory.
+template <int P>
+struct ZWeights {
+    double wz[P];
+};
+
+template <int C, int P>
+__device__ inline void zweights(const Grid& g, const FootSt& f, ZWeights<P>& w) {
     const double* zn = znodes<C>(g);
     const double* dinv = Comp<C>::centers ? g.dinv_cP : g.dinv_fP;
-    double wx[P], wy[P], wz[P];
+    z_weights<P>(zn, dinv, f.z, f.bk, w.wz);
+}
+
+// Order-P gather of one foot given its z weights: the x and y weights, then
+// the staged z -> y -> x contraction with FMAs, from the LDS box when it was
+// staged, else from S.
+template <typename T, int C, int P>
+__device__ inline double contract_w(const Grid& g, const FootSt& f, const ZWeights<P>& w,
+                                    const SBox& b, bool fits, const T* __restrict__ lds,
+                                    int BY, int BZ, const T* __restrict__ S) {
+    double wx[P], wy[P];
     uniform_weights<P>(f.tx, wx);
     uniform_weights<P>(f.ty, wy);
-    z_weights<P>(zn, dinv, f.z, f.bk, wz);
+    const double* wz = w.wz;
     double acc = 0.0;
     if (fits) {
         const int ox = f.bx - b.xmin, oy = f.by - b.ymin, oz = f.bk - b.kmin;
@@ -262,6 +375,16 @@ __device__ inline double contract(const Grid& g, const FootSt& f, const SBox& b,
     return acc;
 }
 
+// Weights, then the contraction.
+template <typename T, int C, int P>
+__device__ inline double contract(const Grid& g, const FootSt& f, const SBox& b, bool fits,
+                                  const T* __restrict__ lds, int BY, int BZ,
+                                  const T* __restrict__ S) {
+    ZWeights<P> w;
+    zweights<C, P>(g, f, w);
+    return contract_w<T, C, P>(g, f, w, b, fits, lds, BY, BZ, S);
+}
+
 __device__ inline void box_add(SBox& l, const FootSt& f) {
     l.xmin = min(l.xmin, f.bx); l.xmax = max(l.xmax, f.bx);
     l.ymin = min(l.ymin, f.by); l.ymax = max(l.ymax, f.by);
@@ -270,14 +393,15 @@ __device__ inline void box_add(SBox& l, const FootSt& f) {
 
 // Iterated midpoint (semilag.py:534-552) from the arrival point with its
 // shared first sample (u0, v0, w0); returns the unclamped foot.
+template <int C>
 __device__ inline void trace_foot(const Grid& g, const TrajA& V, double xa, double ya,
-                                  double za, double u0, double v0, double w0, double h,
+                                  double za, int K, double u0, double v0, double w0, double h,
                                   double& x, double& y, double& z) {
     double xm = xa - h * u0, ym = ya - h * v0;
     double zm = clampd(za - h * w0, g.z_lo, g.z_hi);
     for (int it = 1; it < g.n_iters; ++it) {
         double us, vs, ws;
-        sample3(g, V, xm, ym, zm, us, vs, ws);
+        sample3<C>(g, V, xm, ym, zm, za, K, us, vs, ws);
         xm = xa - h * us;
         ym = ya - h * vs;
         zm = clampd(za - h * ws, g.z_lo, g.z_hi);
@@ -295,12 +419,16 @@ __device__ inline void trace_foot(const Grid& g, const TrajA& V, double xa, doub
 // One tile of the step kernel (tile = linear tile index, k fastest).  The
 // caller provides the block's LDS: lds (LDS_BYTES), sh (6 ints; 12 for
 // MODE_BDF2), s_cnt, and for MODE_BDF2 nst (one FootSt per tile point).
+// defer (MODE_GATHER): when non-null, a tile whose box does not fit the LDS
+// budget is not gathered here: its index is appended to the list
+// defer[1..] (count in defer[0]) for a pass with a larger budget.
 template <typename T, int C, int P, int MODE, int TI, int TJ, int TK, int LDS_BYTES>
 __device__ __forceinline__ void sl_tile_body(
     const Grid& g, const ArrivalTabs& A, const TrajVel& V, const double* __restrict__ F,
     const double* __restrict__ Fm, const T* __restrict__ Sn, const T* __restrict__ Sm,
     double dt, double* __restrict__ out, const Feet& ft, unsigned long long* nclamp, long tile,
-    T* __restrict__ lds, int* sh, int* s_cnt, FootSt* nst = nullptr) {
+    T* __restrict__ lds, int* sh, int* s_cnt, FootSt* nst = nullptr,
+    unsigned* defer = nullptr) {
     constexpr int NPT = TI * TJ * TK;
     static_assert(NPT % kSB == 0, "tile must be a multiple of the block");
     constexpr int PT = NPT / kSB;
@@ -340,12 +468,12 @@ __device__ __forceinline__ void sl_tile_body(
             } else {
                 // the first sample is at the arrival point for every foot depth
                 double u0, v0, w0;
-                sample3_at(g, VA, A.xdx[I - 1], A.ydy[J - 1], za, A.cell[K - 1], u0, v0, w0);
+                sample3_at(g, VA, A.xdx[I - 1], A.ydy[J - 1], za, A.csel[K - 1], u0, v0, w0);
                 if (MODE == MODE_FAR || MODE == MODE_BDF2) {
                     // the near foot first: it leaves the registers (to LDS or
                     // to the feet arrays) before the far foot is traced
                     double xn, yn, zn;
-                    trace_foot(g, VA, xa, ya, za, u0, v0, w0, 0.5 * dt, xn, yn, zn);
+                    trace_foot<C>(g, VA, xa, ya, za, K, u0, v0, w0, 0.5 * dt, xn, yn, zn);
                     if (MODE == MODE_FAR) {
                         ft.x[n] = xn;
                         ft.y[n] = yn;
@@ -357,7 +485,7 @@ __device__ __forceinline__ void sl_tile_body(
                         nst[q] = sn;
                     }
                 }
-                trace_foot(g, VA, xa, ya, za, u0, v0, w0,
+                trace_foot<C>(g, VA, xa, ya, za, K, u0, v0, w0,
                            (MODE == MODE_FAR || MODE == MODE_BDF2) ? 0.5 * (2.0 * dt) : 0.5 * dt,
                            x, y, z);
             }
@@ -367,13 +495,21 @@ __device__ __forceinline__ void sl_tile_body(
     }
 
     // ---- stage the tile's stencil box in LDS -------------------------------
+    // With one point per lane, the z weights are formed before the box is
+    // staged, so their table loads are in flight with the staging's.
+    constexpr bool HOIST = PT == 1;
+    ZWeights<P> w0;
+    if (HOIST && idx[0] >= 0) zweights<C, P>(g, st[0], w0);
     const T* __restrict__ S = (MODE == MODE_FAR || MODE == MODE_BDF2) ? Sm : Sn;
-    // MODE_BDF2: the near box is reduced now and waits in sh[6..11], so no
-    // lane holds it through the far contraction
-    if (MODE == MODE_BDF2) s_box_reduce(ln, sh + 6);
-    const SBox b = s_block_box(l, sh);
+    // MODE_BDF2: the near box is reduced with the far one and waits in
+    // sh[6..11], so no lane holds it through the far contraction
+    const SBox b = (MODE == MODE_BDF2) ? s_block_box2(l, ln, sh) : s_block_box(l, sh);
     int BY = 0, BZ = 0;
     const bool fits = stage_box<T, C, P>(g, S, b, lds, LDS_BYTES, BY, BZ);
+    if (MODE == MODE_GATHER && defer && !fits) {  // uniform over the block
+        if (threadIdx.x == 0) defer[1 + atomicAdd(defer, 1u)] = (unsigned)tile;
+        return;
+    }
 
     // ---- phase 2: contract -------------------------------------------------
     const double rdx2 = 1.0 / (g.dx * g.dx), rdy2 = 1.0 / (g.dy * g.dy);
@@ -382,8 +518,12 @@ __device__ __forceinline__ void sl_tile_body(
         // far value of every point, then the near box into the same buffer
 #pragma unroll
         for (int p = 0; p < PT; ++p)
-            farv[p] = idx[p] < 0 ? 0.0 : contract<T, C, P>(g, st[p], b, fits, lds, BY, BZ, S);
+            farv[p] = idx[p] < 0 ? 0.0
+                      : HOIST ? contract_w<T, C, P>(g, st[p], w0, b, fits, lds, BY, BZ, S)
+                              : contract<T, C, P>(g, st[p], b, fits, lds, BY, BZ, S);
         __syncthreads();  // every lane is done with the far box
+        // the near stencils' weights, before the near box is staged
+        if (HOIST && idx[0] >= 0) zweights<C, P>(g, nst[threadIdx.x], w0);
     }
     const SBox bn = (MODE == MODE_BDF2) ? SBox{sh[6], sh[7], sh[8], sh[9], sh[10], sh[11]} : b;
     const bool fits_n = (MODE == MODE_BDF2) ? stage_box<T, C, P>(g, Sn, bn, lds, LDS_BYTES, BY, BZ)
@@ -393,7 +533,11 @@ __device__ __forceinline__ void sl_tile_body(
         if (idx[p] < 0) continue;
         const long n = idx[p];
         const double acc =
-            (MODE == MODE_BDF2)
+            HOIST ? ((MODE == MODE_BDF2)
+                         ? contract_w<T, C, P>(g, nst[threadIdx.x], w0, bn, fits_n, lds, BY, BZ,
+                                               Sn)
+                         : contract_w<T, C, P>(g, st[p], w0, b, fits, lds, BY, BZ, S))
+            : (MODE == MODE_BDF2)
                 ? contract<T, C, P>(g, nst[threadIdx.x + kSB * p], bn, fits_n, lds, BY, BZ, Sn)
                 : contract<T, C, P>(g, st[p], b, fits, lds, BY, BZ, S);
         // arrival ghost index of n
