@@ -294,7 +294,7 @@ int slc::plan_create(const slc_grid_desc* d, int device, bool spectral, slc_plan
         {
             long tiles = 0;
             for (int c = 0; c < 3; ++c) tiles = std::max(tiles, gather_tiles(g, c));
-            if ((rc = dev_alloc(p, sizeof(unsigned) * (size_t)(tiles + 1), &ptr))) {
+            if ((rc = dev_alloc(p, sizeof(unsigned) * (size_t)tiles, &ptr))) {
                 slc_plan_destroy(p);
                 return rc;
             }

@@ -53,15 +53,15 @@ struct ArrivalTabs {
 // LDS-tiled step kernels (slc_step.hip); mode: 0 advect, 1 bdf1, 2 far,
 // 3 near, 4 gather (feet given), 5 far + near in one pass (no feet arrays)
 // Fn32 / Fm32: float copies of F / Fm to gather from (FP32 storage), or null.
-// defer (mode 4 only): 1 + (tiles of the component) unsigned of device
-// scratch; when given, tiles whose box exceeds the LDS budget are gathered
-// by a second pass with a larger budget instead of by direct loads.
+// defer (mode 4 only): (tiles of the component) unsigned of device scratch,
+// one flag per tile; when given, tiles whose box exceeds the LDS budget are
+// gathered by a second pass with a larger budget instead of by direct loads.
 hipError_t run_tile_step(const Grid& g, int comp, int mode, const ArrivalTabs& A,
                          const TrajVel& V, const double* F, const double* Fm, const float* Fn32,
                          const float* Fm32, double dt, double* out, double* fx, double* fy,
                          double* fz, unsigned long long* ncl, hipStream_t s,
                          unsigned* defer = nullptr);
-// Tiles of the gather at given feet for component comp (the defer list's size).
+// Tiles of the gather at given feet for component comp (the defer flags' count).
 long gather_tiles(const Grid& g, int comp);
 hipError_t run_arrival_tabs(const Grid& g, int comp, double* xdx, double* ydy, int* cell,
                             hipStream_t s);

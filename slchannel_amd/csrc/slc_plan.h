@@ -71,7 +71,7 @@ struct slc_plan {
     double* diag_part = nullptr;  // slc_diagnostics partials (5 per workgroup)
     double* us[3] = {nullptr, nullptr, nullptr};  // U* = 2V^n - V^{n-1} (ghost-shaped)
     double* nfoot = nullptr;                      // near feet of one component (3 x N)
-    unsigned* defer = nullptr;  // slc_gather's list of tiles for the wide second pass
+    unsigned* defer = nullptr;  // slc_gather's per-tile flags for the wide second pass
     ArrivalTabs at[3] = {};
     // opt-in launch timing: hipEvents recorded around each labelled launch
     bool prof = false;

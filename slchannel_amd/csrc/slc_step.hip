@@ -51,28 +51,27 @@ __global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
 }
 
 // The gather at given feet for the tiles a first pass deferred (their boxes
-// exceed its LDS budget): the tile indices are read from list[1..list[0]];
-// blocks beyond the count exit.  A tile that does not fit this budget
-// either takes the direct-load path.  gfx950 gives one workgroup up to
-// 160 KB of LDS; 76 KB keeps two of these blocks per CU and holds the box of
-// a 4x4x16 tile whose feet spread over +-5 cells in x and y at order 6
+// exceed its LDS budget; flags[tile] = 1); the blocks of the other tiles exit.
+// Same tile order as the first pass (xcd_block).  A tile that does not fit
+// this budget either takes the direct-load path.  gfx950 gives one workgroup
+// up to 160 KB of LDS; 76 KB keeps two of these blocks per CU and holds the
+// box of a 4x4x16 tile whose feet spread over +-5 cells in x and y at order 6
 // (at most 20 x 20 x 21 doubles and the row bases).
 constexpr int kWideLds = 77824;
 
 template <int C, int P>
 __global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(2))) void gather_wide_kernel(
     Grid g, ArrivalTabs A, const double* __restrict__ F, Feet ft, double* __restrict__ out,
-    unsigned long long* nclamp, const unsigned* __restrict__ list) {
+    unsigned long long* nclamp, const unsigned* __restrict__ flags) {
     __shared__ double lds[kWideLds / sizeof(double)];
     __shared__ int sh[6];
     __shared__ int s_cnt;
-    if (blockIdx.x >= list[0]) return;
+    const long tile = xcd_block(gridDim.x);
+    if (flags[tile] == 0u) return;
     const TrajVel V{{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
     sl_tile_body<double, C, P, MODE_GATHER, SLC_GTILE_TI, SLC_GTILE_TJ, SLC_GTILE_TK, kWideLds>(
-        g, A, V, F, F, F, F, 0.0, out, ft, nclamp, (long)list[1 + blockIdx.x], lds, sh, &s_cnt);
+        g, A, V, F, F, F, F, 0.0, out, ft, nclamp, tile, lds, sh, &s_cnt);
 }
-
-__global__ void zero_u32_kernel(unsigned* p) { *p = 0u; }
 
 // Arrival tables of component C: x/dx, y/dy of the arrival columns and the
 // face cell and centre split of each arrival z (2 c + split), evaluated with
@@ -198,7 +197,7 @@ struct StepArgs {
     double* out;
     Feet ft;
     unsigned long long* ncl;
-    unsigned* defer;  // MODE_GATHER: tile list of the wide second pass, or null
+    unsigned* defer;  // MODE_GATHER: per-tile flags of the wide second pass, or null
 };
 
 template <typename T, int C, int P, int MODE>
@@ -214,7 +213,6 @@ static void launch_mode(const Grid& g, const ArrivalTabs& A, const TrajVel& V,
                                       ((g.ny + TJ - 1) / TJ) * ((nzi + TK - 1) / TK));
     if (tiles == 0) return;
     unsigned* defer = (MODE == MODE_GATHER) ? a.defer : nullptr;
-    if (defer) hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, s, defer);
     hipLaunchKernelGGL((sl_tile_kernel<T, C, P, MODE, TI, TJ, TK, LDS, W>), dim3(tiles),
                        dim3(kSB), 0, s, g, A, V, a.F, a.Fm, (const T*)a.Sn, (const T*)a.Sm, a.dt,
                        a.out, a.ft, a.ncl, defer);

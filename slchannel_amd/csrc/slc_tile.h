@@ -419,9 +419,9 @@ __device__ inline void trace_foot(const Grid& g, const TrajA& V, double xa, doub
 // One tile of the step kernel (tile = linear tile index, k fastest).  The
 // caller provides the block's LDS: lds (LDS_BYTES), sh (6 ints; 12 for
 // MODE_BDF2), s_cnt, and for MODE_BDF2 nst (one FootSt per tile point).
-// defer (MODE_GATHER): when non-null, a tile whose box does not fit the LDS
-// budget is not gathered here: its index is appended to the list
-// defer[1..] (count in defer[0]) for a pass with a larger budget.
+// defer (MODE_GATHER): when non-null, one flag per tile: 1 for a tile whose
+// box does not fit the LDS budget (it is not gathered here but left for a
+// pass with a larger budget), 0 for the others.
 template <typename T, int C, int P, int MODE, int TI, int TJ, int TK, int LDS_BYTES>
 __device__ __forceinline__ void sl_tile_body(
     const Grid& g, const ArrivalTabs& A, const TrajVel& V, const double* __restrict__ F,
@@ -506,9 +506,9 @@ __device__ __forceinline__ void sl_tile_body(
     const SBox b = (MODE == MODE_BDF2) ? s_block_box2(l, ln, sh) : s_block_box(l, sh);
     int BY = 0, BZ = 0;
     const bool fits = stage_box<T, C, P>(g, S, b, lds, LDS_BYTES, BY, BZ);
-    if (MODE == MODE_GATHER && defer && !fits) {  // uniform over the block
-        if (threadIdx.x == 0) defer[1 + atomicAdd(defer, 1u)] = (unsigned)tile;
-        return;
+    if (MODE == MODE_GATHER && defer) {  // fits is uniform over the block
+        if (threadIdx.x == 0) defer[tile] = fits ? 0u : 1u;
+        if (!fits) return;
     }
 
     // ---- phase 2: contract -------------------------------------------------
