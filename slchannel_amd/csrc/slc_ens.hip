@@ -62,8 +62,7 @@ struct Member {
 };
 
 // members of one grid size share plane-major spectral buffers (one slot per
-// member) and batched rocFFT plans; transforms of k members run as chunks of
-// power-of-two batch sizes
+// member) and one batched rocFFT plan per direction over all the slots
 struct SizeClass {
     int nx, ny, nz;
     int count = 0;     // members in the table
@@ -71,7 +70,7 @@ struct SizeClass {
     double* planes = nullptr;
     double* dhat = nullptr;
     double* cpw = nullptr;
-    std::map<int, rocfft_plan> fwd, inv;  // by batch chunk (members)
+    std::map<int, rocfft_plan> fwd, inv;  // by batch (slots); only the current capacity's
 };
 
 constexpr int kFieldSets = 3;
@@ -187,47 +186,46 @@ void free_class(SizeClass& c) {
     c.cap = 0;
 }
 
-// power-of-two chunks of k members, largest first
-std::vector<int> chunks(int k) {
-    std::vector<int> out;
-    for (int b = 1 << 30; b > 0; b >>= 1)
-        if (k & b) out.push_back(b);
-    return out;
-}
-
+// The class's transforms: one forward and one inverse plan over all its
+// slots (cap members x nz planes), so a step executes one plan per class and
+// direction whatever the member count; slots beyond the count are
+// transformed too and never read.  Plans of an earlier capacity are dropped.
 int class_plans(slc_ens* e, SizeClass& c) {
-    for (int q : chunks(c.count)) {
-        if (c.fwd.count(q)) continue;
-        size_t lengths[2] = {(size_t)c.ny, (size_t)c.nx};
-        rocfft_plan f = nullptr, b = nullptr;
-        rocfft_status s = rocfft_plan_create(&f, rocfft_placement_notinplace,
-                                             rocfft_transform_type_real_forward,
-                                             rocfft_precision_double, 2, lengths,
-                                             (size_t)q * c.nz, nullptr);
-        if (s != rocfft_status_success)
-            return fail(SLC_EFFT, "rocfft_plan_create(forward, batch %d) status %d", q * c.nz,
-                        (int)s);
-        s = rocfft_plan_create(&b, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
-                               rocfft_precision_double, 2, lengths, (size_t)q * c.nz, nullptr);
-        if (s != rocfft_status_success) {
-            rocfft_plan_destroy(f);
-            return fail(SLC_EFFT, "rocfft_plan_create(inverse, batch %d) status %d", q * c.nz,
-                        (int)s);
-        }
-        c.fwd[q] = f;
-        c.inv[q] = b;
-        size_t w1 = 0, w2 = 0;
-        rocfft_plan_get_work_buffer_size(f, &w1);
-        rocfft_plan_get_work_buffer_size(b, &w2);
-        const size_t w = std::max(w1, w2);
-        if (w > e->fft_work_bytes) {
-            if (e->fft_work) (void)hipFree(e->fft_work);
-            e->fft_work = nullptr;
-            e->fft_work_bytes = 0;
-            HIP_OK(hipMalloc(&e->fft_work, w));
-            e->fft_work_bytes = w;
-            FFT_OK(rocfft_execution_info_set_work_buffer(e->info, e->fft_work, w));
-        }
+    const int q = c.cap;
+    if (c.fwd.count(q)) return SLC_OK;
+    for (auto& kv : c.fwd) rocfft_plan_destroy(kv.second);
+    for (auto& kv : c.inv) rocfft_plan_destroy(kv.second);
+    c.fwd.clear();
+    c.inv.clear();
+    size_t lengths[2] = {(size_t)c.ny, (size_t)c.nx};
+    rocfft_plan f = nullptr, b = nullptr;
+    rocfft_status s = rocfft_plan_create(&f, rocfft_placement_notinplace,
+                                         rocfft_transform_type_real_forward,
+                                         rocfft_precision_double, 2, lengths,
+                                         (size_t)q * c.nz, nullptr);
+    if (s != rocfft_status_success)
+        return fail(SLC_EFFT, "rocfft_plan_create(forward, batch %d) status %d", q * c.nz,
+                    (int)s);
+    s = rocfft_plan_create(&b, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
+                           rocfft_precision_double, 2, lengths, (size_t)q * c.nz, nullptr);
+    if (s != rocfft_status_success) {
+        rocfft_plan_destroy(f);
+        return fail(SLC_EFFT, "rocfft_plan_create(inverse, batch %d) status %d", q * c.nz,
+                    (int)s);
+    }
+    c.fwd[q] = f;
+    c.inv[q] = b;
+    size_t w1 = 0, w2 = 0;
+    rocfft_plan_get_work_buffer_size(f, &w1);
+    rocfft_plan_get_work_buffer_size(b, &w2);
+    const size_t w = std::max(w1, w2);
+    if (w > e->fft_work_bytes) {
+        if (e->fft_work) (void)hipFree(e->fft_work);
+        e->fft_work = nullptr;
+        e->fft_work_bytes = 0;
+        HIP_OK(hipMalloc(&e->fft_work, w));
+        e->fft_work_bytes = w;
+        FFT_OK(rocfft_execution_info_set_work_buffer(e->info, e->fft_work, w));
     }
     return SLC_OK;
 }
@@ -439,17 +437,9 @@ int fft_all(slc_ens* e, bool forward) {
     FFT_OK(rocfft_execution_info_set_stream(e->info, e->es));
     for (SizeClass& c : e->classes) {
         if (c.count == 0) continue;
-        const long nky = c.ny / 2 + 1;
-        const size_t per = (size_t)c.nx * c.ny * c.nz, perk = (size_t)c.nx * nky * c.nz;
-        int s0 = 0;
-        for (int q : chunks(c.count)) {
-            double* pl = c.planes + (size_t)s0 * per;
-            double* dh = c.dhat + 2 * (size_t)s0 * perk;
-            void* in[1] = {forward ? (void*)pl : (void*)dh};
-            void* out[1] = {forward ? (void*)dh : (void*)pl};
-            FFT_OK(rocfft_execute(forward ? c.fwd[q] : c.inv[q], in, out, e->info));
-            s0 += q;
-        }
+        void* in[1] = {forward ? (void*)c.planes : (void*)c.dhat};
+        void* out[1] = {forward ? (void*)c.dhat : (void*)c.planes};
+        FFT_OK(rocfft_execute(forward ? c.fwd[c.cap] : c.inv[c.cap], in, out, e->info));
     }
     return SLC_OK;
 }
