@@ -214,6 +214,28 @@ int slc_ab2_update(slc_plan* plan, double* u, double* v, double* w, const double
 int slc_diagnostics(slc_plan* plan, const double* u, const double* v, const double* w,
                     const double* p, double dt, double* out, void* stream);
 
+/* Seeded solenoidal initial velocity on the device (no counterpart in the
+ * reference, whose initialisers are the parabolic / vortex / uniform fields of
+ * initflow.py:123-223; the seeded perturbation of SLChannelFlow.initialize is
+ * this build's own).  Writes the interiors of u, v, w (ghost-shaped; ghosts
+ * untouched): the curl of A_c = sum_b Re[a^c_b e^{i(kx x + ky y)}] G_l(z)
+ * over W x W wavenumber bins (W odd, bins (m, n) with |m|, |n| <= (W-1)/2)
+ * and l = 1..L, scaled to the given u'rms over the three interiors, plus
+ * the base profile on u.  Device inputs:
+ *   ab     five (W*W, L) complex blocks (re, im interleaved): the
+ *          coefficients of G_l and dG_l/dz per bin -- alpha_u, beta_u,
+ *          alpha_v, beta_v, alpha_w;
+ *   zt     G_l at the interior centres (L, nz), dG_l/dz there (L, nz), G_l at
+ *          the interior faces (L, nz-1), the base profile of u (nz);
+ *   phase  per component (u, v, w) the x factors e^{i kx_m x_i} (W, nx), then
+ *          the y factors e^{i ky_n y_j} (W, ny), at its staggered positions;
+ *   work   scratch of at least slc_init_solenoidal_work(nx, ny, nz, W) bytes.
+ * Four launches on the stream, no host synchronisation. */
+int slc_init_solenoidal(slc_plan* plan, int W, int L, const double* ab, const double* zt,
+                        const double* phase, double urms, double* work, int64_t work_bytes,
+                        double* u, double* v, double* w, void* stream);
+int64_t slc_init_solenoidal_work(int nx, int ny, int nz, int W);
+
 /* Exact bulk-flux constraint: u += U_bulk - ub(u) on the interior, ghosts of
  * u refreshed.  out (4 device doubles): [0] correction, [1] forcing =
  * correction/dt, [2] bulk velocity after the shift, [3] before it.

@@ -18,7 +18,7 @@ BUILD = os.path.join(HERE, "_build")
 SOURCES = ["slc_api.hip", "slc_sl.hip", "slc_ops.hip", "slc_tiled.hip", "slc_step.hip",
            "slc_ens.hip", "slc_ens_kernels.hip", "slc_ens_tile_d4.hip", "slc_ens_tile_d6.hip",
            "slc_ens_tile_f4.hip", "slc_ens_tile_f6.hip", "slc_stats.hip",
-           "slc_diag.hip", "slc_imex.hip"]
+           "slc_diag.hip", "slc_imex.hip", "slc_init.hip"]
 ARCH = os.environ.get("SLC_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -1,4 +1,5 @@
-"""Seeded solenoidal initial fields for the channel (host side, numpy).
+"""Seeded solenoidal initial fields for the channel: numpy on the host, or on
+the device through slc_init_solenoidal (the plan's library).
 
 The reference has no seeded divergence-free random initialiser
 (slchannel/initflow.py:123-223 offers parabolic / vortices / uniform), so this
@@ -42,11 +43,12 @@ def _modes(kmin, kmax, nx, ny):
 
 
 def solenoidal_fields(nx, ny, nz, Lx, Ly, Lz, z_f, z_c, seed, urms=0.1, kmin=1.0, kmax=8.0,
-                      spectrum="flat", U_bulk=1.0, chunk=32, device=None):
-    """Return ghost-shaped arrays (u, v, w) with interiors set: numpy arrays,
-    or float64 torch tensors on `device` when one is given (the x-y
-    synthesis then runs as torch FFTs there; same modes and amplitudes, FFT
-    rounding aside)."""
+                      spectrum="flat", U_bulk=1.0, chunk=32, plan=None):
+    """Return ghost-shaped arrays (u, v, w) with interiors set: numpy arrays
+    (x-y synthesis by numpy FFTs), or, when a plan is given, float64 device
+    tensors on the plan's device built by slc_init_solenoidal (direct
+    summation over the same modes and amplitudes; rounding aside the same
+    fields)."""
     z_f = np.asarray(z_f, dtype=np.float64)
     z_c = np.asarray(z_c, dtype=np.float64)
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -70,9 +72,9 @@ def solenoidal_fields(nx, ny, nz, Lx, Ly, Lz, z_f, z_c, seed, urms=0.1, kmin=1.0
         dg = denv[None, :] * s + env[None, :] * c * (np.outer(ls, np.ones_like(z)) * math.pi / Lz)
         return g, dg
 
-    if device is not None:
-        return _solenoidal_torch(nx, ny, nz, Lx, Ly, Lz, z_f, z_c, urms, U_bulk, device, mm, nn,
-                                 ll, K, L, A, G, dx, dy)
+    if plan is not None:
+        return _solenoidal_hip(plan, nx, ny, nz, Lx, Ly, Lz, z_f, z_c, urms, U_bulk, mm, nn, ll,
+                               K, L, A, G, dx, dy)
     kx = 2 * math.pi * mm / Lx
     ky = 2 * math.pi * nn / Ly
 
@@ -118,87 +120,68 @@ def solenoidal_fields(nx, ny, nz, Lx, Ly, Lz, z_f, z_c, seed, urms=0.1, kmin=1.0
     return u, v, w
 
 
-def _solenoidal_torch(nx, ny, nz, Lx, Ly, Lz, z_f, z_c, urms, U_bulk, device, mm, nn, ll, K, L,
-                      A, G, dx, dy):
-    """The same fields on the device: the amplitudes go into dense
-    (m, n, l) arrays in one upload; per component and z chunk, one complex
-    matrix product sums the l modes of every (m, n) bin, the bins are placed
-    in the spectral plane (no two bins share a wavenumber, so nothing is
-    accumulated and the sums do not depend on launch order), and one batched
-    inverse FFT gives the planes.  No host synchronisation."""
+_HIP_TABLES = {}
+
+
+def _solenoidal_hip(plan, nx, ny, nz, Lx, Ly, Lz, z_f, z_c, urms, U_bulk, mm, nn, ll, K, L, A,
+                    G, dx, dy):
+    """The fields on the plan's device with slc_init_solenoidal.  Per request
+    the host draws nothing new here (the amplitudes A are given) and uploads
+    the per-bin coefficients of G_l and dG_l/dz (five (W^2, L) complex blocks,
+    one pinned copy); the z tables and the staggered-position phase factors
+    depend only on the grid and the mode box and are built once per grid.
+    Four kernel launches, no host synchronisation."""
     import torch
 
-    dev = torch.device(device)
+    import hashlib
+
+    dev = plan.device
     W = 2 * K + 1
-    ks = np.arange(-K, K + 1)
-    kxb = np.repeat(2 * math.pi * ks / Lx, W)   # bin b = (m + K) W + (n + K)
-    kyb = np.tile(2 * math.pi * ks / Ly, W)
+    zdig = hashlib.sha1(np.ascontiguousarray(z_c, dtype=np.float64).tobytes()
+                        + np.ascontiguousarray(z_f, dtype=np.float64).tobytes()).hexdigest()
+    key = (str(dev), nx, ny, nz, Lx, Ly, Lz, zdig, K, L, float(U_bulk))
+    tabs = _HIP_TABLES.get(key)
+    if tabs is None:
+        zc_in = np.asarray(z_c[1 : nz + 1], dtype=np.float64)
+        zf_in = np.asarray(z_f[1:nz], dtype=np.float64)
+        ls = np.arange(1, L + 1)
+        gc, dgc = G(zc_in, ls)
+        gf, _ = G(zf_in, ls)
+        zeta = 2.0 * zc_in / Lz - 1.0
+        base = 1.5 * U_bulk * (1.0 - zeta**2)
+        zt = np.concatenate([gc.ravel(), dgc.ravel(), gf.ravel(), base])
+        ks = np.arange(-K, K + 1)
+        kx, ky = 2 * math.pi * ks / Lx, 2 * math.pi * ks / Ly
+        xi, yj = np.arange(nx) * dx, np.arange(ny) * dy
+        ph = []
+        # u at (i dx + dx, j dy + dy/2), v at (i dx + dx/2, j dy + dy), w at
+        # (i dx + dx/2, j dy + dy/2): array index i <-> ghost index i + 1
+        for sx, sy in ((dx, 0.5 * dy), (0.5 * dx, dy), (0.5 * dx, 0.5 * dy)):
+            ph.append(np.exp(1j * np.outer(kx, xi + sx)).ravel().view(np.float64))
+            ph.append(np.exp(1j * np.outer(ky, yj + sy)).ravel().view(np.float64))
+        tabs = (torch.from_numpy(zt).to(dev), torch.from_numpy(np.concatenate(ph)).to(dev))
+        _HIP_TABLES[key] = tabs
+    ztd, phd = tabs
 
     def dense(a):
         d = np.zeros((W, W, L), dtype=np.complex128)
         d[mm + K, nn + K, ll - 1] = a
         return d.reshape(W * W, L)
 
+    ks = np.arange(-K, K + 1)
+    ikx = (1j * np.repeat(2 * math.pi * ks / Lx, W))[:, None]   # bin b = (m + K) W + (n + K)
+    iky = (1j * np.tile(2 * math.pi * ks / Ly, W))[:, None]
     Ax, Ay, Az = dense(A["x"]), dense(A["y"]), dense(A["z"])
-    ikx, iky = (1j * kxb)[:, None], (1j * kyb)[:, None]
     # u = dAz/dy - dAy/dz ; v = dAx/dz - dAz/dx ; w = dAy/dx - dAx/dy: the
     # coefficients of g_l (alpha) and of dg_l/dz (beta) per bin and l
-    alpha_beta = [iky * Az, -Ay, -ikx * Az, Ax, ikx * Ay - iky * Ax]
-    zc_in = np.asarray(z_c[1 : nz + 1], dtype=np.float64)
-    zf_in = np.asarray(z_f[1:nz], dtype=np.float64)
-    gc, dgc = G(zc_in, np.arange(1, L + 1))
-    gf, _ = G(zf_in, np.arange(1, L + 1))
-    zeta = 2.0 * zc_in / Lz - 1.0
-    base = 1.5 * U_bulk * (1.0 - zeta**2)
-    shifts = [(dx, 0.5 * dy), (0.5 * dx, dy), (0.5 * dx, 0.5 * dy)]  # u, v, w positions
-    phases = [np.exp(1j * (kxb * sx + kyb * sy)) for sx, sy in shifts]
-    bins = np.concatenate([ks.repeat(W) % nx, np.tile(ks, W) % ny]).astype(np.float64)
-    # one pinned upload of everything the synthesis reads
-    cplx = alpha_beta + phases
-    host = np.concatenate([x.ravel().view(np.float64) for x in cplx]
-                          + [gc.ravel(), dgc.ravel(), gf.ravel(), base, bins])
-    buf = torch.from_numpy(host)
-    buf = buf.pin_memory().to(dev, non_blocking=True) if dev.type == "cuda" else buf.to(dev)
-    parts, off = [], 0
-    for x in cplx:
-        parts.append(torch.view_as_complex(buf[off:off + 2 * x.size].view(-1, 2)).view(x.shape))
-        off += 2 * x.size
-    au, bu, av, bv, aw, pu, pv, pw = parts
-    gct = buf[off:off + gc.size].view(gc.shape); off += gc.size
-    dgct = buf[off:off + dgc.size].view(dgc.shape); off += dgc.size
-    gft = buf[off:off + gf.size].view(gf.shape); off += gf.size
-    baset = buf[off:off + nz]; off += nz
-    mi = buf[off:off + W * W].long(); off += W * W
-    ni = buf[off:off + W * W].long()
-    # z chunks of at most 2^27 plane points (2 GiB of complex spectrum)
-    chunk = max(1, min(nz, (1 << 27) // (nx * ny)))
-
-    def evaluate(alpha, beta, g, dg, phase):
-        nzs = g.shape[1]
-        out = torch.empty((nx, ny, nzs), dtype=torch.float64, device=dev)
-        for z0 in range(0, nzs, chunk):
-            z1 = min(nzs, z0 + chunk)
-            coef = alpha @ g[:, z0:z1].to(torch.complex128)
-            if beta is not None:
-                coef = coef + beta @ dg[:, z0:z1].to(torch.complex128)
-            C = torch.zeros((nx, ny, z1 - z0), dtype=torch.complex128, device=dev)
-            C[mi, ni] = coef * phase[:, None]
-            out[:, :, z0:z1] = torch.fft.ifft2(C, dim=(0, 1)).real * (nx * ny)
-        return out
-
-    up = evaluate(au, bu, gct, dgct, pu)
-    vp = evaluate(av, bv, gct, dgct, pv)
-    wp = evaluate(aw, None, gft, None, pw)
-    # the scale stays on the device (no host synchronisation): the same IEEE
-    # operations as urms / math.sqrt(sum / count) on the host
-    ss = ((up**2).sum() + (vp**2).sum() + (wp**2).sum()) / (up.numel() + vp.numel()
-                                                            + wp.numel())
-    scale = torch.where(ss > 0, urms / torch.sqrt(ss), torch.zeros_like(ss))
+    ab = np.concatenate([x.ravel() for x in (iky * Az, -Ay, -ikx * Az, Ax, ikx * Ay - iky * Ax)])
+    buf = torch.from_numpy(ab.view(np.float64))
+    abd = buf.pin_memory().to(dev, non_blocking=True) if dev.type == "cuda" else buf.to(dev)
     f64 = dict(dtype=torch.float64, device=dev)
+    # scratch from the caching allocator, returned when the call's work is done
+    work = torch.empty((plan.init_solenoidal_work(W) + 7) // 8, **f64)
     u = torch.zeros((nx + 1, ny + 2, nz + 2), **f64)
     v = torch.zeros((nx + 2, ny + 1, nz + 2), **f64)
     w = torch.zeros((nx + 2, ny + 2, nz + 1), **f64)
-    u[1 : nx + 1, 1 : ny + 1, 1 : nz + 1] = scale * up + baset[None, None, :]
-    v[1 : nx + 1, 1 : ny + 1, 1 : nz + 1] = scale * vp
-    w[1 : nx + 1, 1 : ny + 1, 1:nz] = scale * wp
+    plan.init_solenoidal(W, L, abd, ztd, phd, urms, work, u, v, w)
     return u, v, w

@@ -307,6 +307,24 @@ class HipPlan:
                                             *[_ptr(t) for t in r], *rp, float(dt),
                                             self._stream()), "slc_ab2_update")
 
+    def init_solenoidal(self, W, L, ab, zt, phase, urms, work, u, v, w):
+        """Seeded solenoidal interiors of u, v, w (slc_init_solenoidal): ab, zt,
+        phase and work are float64 device tensors laid out as include/slc.h
+        describes; ghosts are not written."""
+        self._uvw(u, v, w)
+        for t, name in ((ab, "ab"), (zt, "zt"), (phase, "phase"), (work, "work")):
+            if not (t.is_contiguous() and t.dtype == torch.float64 and t.device == self.device):
+                raise ValueError(f"init_solenoidal: {name} must be a contiguous float64 tensor "
+                                 f"on {self.device}")
+        _lib.check(self._lib.slc_init_solenoidal(
+            self._h, int(W), int(L), _ptr(ab), _ptr(zt), _ptr(phase), float(urms), _ptr(work),
+            int(work.numel() * 8), _ptr(u), _ptr(v), _ptr(w), self._stream()),
+            "slc_init_solenoidal")
+
+    def init_solenoidal_work(self, W):
+        """Scratch bytes slc_init_solenoidal needs for W x W bins."""
+        return int(self._lib.slc_init_solenoidal_work(self.nx, self.ny, self.nz, int(W)))
+
     def diagnostics(self, u, v, w, p, dt, out):
         """out (5 device doubles): max|div|, u_tau, CFL = dti*dt, dti, number
         of non-finite entries of u, v, w, p (p may be None)."""

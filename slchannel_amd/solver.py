@@ -191,7 +191,7 @@ class SLChannelFlow:
         u, v, w = solenoidal_fields(self.nx, self.ny, self.nz, self.Lx, self.Ly, self.Lz,
                                     self.z_f.numpy(), self.z_c.numpy(), seed, urms=urms,
                                     kmin=kmin, kmax=kmax, spectrum=spectrum, U_bulk=self.U_bulk,
-                                    device=self.device)
+                                    plan=self.plan)
         self.set_fields(u, v, w)
         self.apply_bc_uvw()
         self.project_initial()
