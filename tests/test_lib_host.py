@@ -22,7 +22,7 @@ def declared_symbols():
     out = set()
     for h in ("slc.h", "slc_ens.h", "slc_stats.h"):
         hdr = open(os.path.join(REPO, "include", h)).read()
-        out |= set(re.findall(r"^(?:int|const char\*)\s+(slc_\w+)\(", hdr, re.M))
+        out |= set(re.findall(r"^(?:int|int64_t|const char\*)\s+(slc_\w+)\(", hdr, re.M))
     return out
 
 
