@@ -595,8 +595,10 @@ def serve_bench(args):
     {64^3, 128x96x128, 256^3, 384x384x256}, Re ~ U[2800, 10000], N ~ U[20, 200]
     steps, order 6, dt for trajectory CFL 3; admission by HBM footprint under
     the given budget.  The plain per-run path is each request run alone
-    (SLChannelFlow), served first-come-first-served under the same arrival
-    times (its queue simulated from the measured stand-alone durations)."""
+    (SLChannelFlow: its seeded initial state, then its steps; solver plans
+    built beforehand, as the server's per-grid initialisers are), served
+    first-come-first-served under the same arrival times (its queue simulated
+    from the measured stand-alone durations)."""
     import numpy as np
 
     from slchannel_amd.serve import Server, plain_fifo, poisson_requests, run_alone
