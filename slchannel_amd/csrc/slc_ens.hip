@@ -366,7 +366,10 @@ int rebuild(slc_ens* e) {
                                                           zsolve_ld(k, m->nz) * sizeof(double));
         }
         units[EK_PLANE] = (unsigned)plane_tiles(m->nx, m->ny, m->nz);
-        units[EK_THOMAS] = ublocks((long)m->nx * nky);
+        units[EK_THOMAS] = spike_ok(m->nz)
+                               ? (unsigned)(((long)m->nx * nky + spike_modes(m->nz) - 1) /
+                                            spike_modes(m->nz))
+                               : ublocks((long)m->nx * nky);
         units[EK_BULK] = (unsigned)M.npart;
         units[EK_ONE] = 1;
         for (int k = 0; k < EK_COUNT; ++k)

@@ -185,11 +185,18 @@ hipError_t run_ens_div_planes(const EnsLaunch& L, unsigned grid, hipStream_t s) 
     return hipGetLastError();
 }
 
+// one unit = one block of the single-run solve (the partition solve of
+// spike_modes(nz) modes, or 256 modes of the sequential sweep), so members
+// get the single run's doubles
 __global__ __launch_bounds__(256) void ens_thomas_kernel(ENS_PARAMS) {
+    __shared__ double lds[256 * 8];
     ens_for(start, nmem, total, [&](int m, unsigned u) {
         const EnsMember& M = mem[m];
-        poisson_thomas_body(M.pt, reinterpret_cast<double2*>(M.dhat), M.cpw,
-                            (long)u * 256 + threadIdx.x);
+        if (spike_ok(M.pt.nz))
+            poisson_spike_body(M.pt, reinterpret_cast<double2*>(M.dhat), (long)u, lds);
+        else
+            poisson_thomas_body(M.pt, reinterpret_cast<double2*>(M.dhat), M.cpw,
+                                (long)u * 256 + threadIdx.x);
     });
 }
 

@@ -561,6 +561,185 @@ __device__ inline void poisson_thomas_body(const PoissonTabs& t, double2* __rest
 }
 
 // ---------------------------------------------------------------------------
+// per-wavenumber tridiagonal solve by partition (the system of
+// poisson_thomas_body, solved in another order: 32 B of memory traffic per
+// complex unknown instead of 80)
+// ---------------------------------------------------------------------------
+// A 256-thread block takes spike_modes(nz) consecutive modes; its threads
+// split each mode's nz levels into spike_segs(nz) segments of at most
+// kSpikeR levels (thread t: mode t % MB, segment t / MB), so each plane row
+// of the block is one contiguous run of MB complex values.  Per segment the
+// thread solves, in registers, the segment's rows with the couplings to its
+// neighbours removed, for the right-hand side (y) and for the two couplings
+// (alpha: unit value one level below the segment, beta: one level above):
+//   x = y + alpha L_prev + beta F_next,
+// L_prev the last level of the segment below and F_next the first of the
+// one above.  One thread per mode then solves the reduced system of the
+// segments' first and last levels (a scalar two-term recurrence across the
+// segments, forward then back), and every thread forms its levels.  Rows,
+// the bottom Neumann row and the (0,0) mode's identity row are those of
+// poisson_thomas_body (projection.py:47-114); the partition is exact
+// algebra, so the solution agrees to rounding.  Needs nz <= 8 * 256; LDS:
+// 256 x 8 doubles.
+constexpr int kSpikeR = 8;
+
+__host__ __device__ inline int spike_segs(int nz) {
+    int s = 32;
+    while (s * kSpikeR < nz) s *= 2;
+    return s;
+}
+
+__host__ __device__ inline int spike_modes(int nz) { return 256 / spike_segs(nz); }
+
+__host__ __device__ inline bool spike_ok(int nz) { return nz <= kSpikeR * 256; }
+
+// row k of the Poisson system of mode m (khsq = kx^2 + ky^2): a (below),
+// b (diagonal), c (above), as poisson_thomas_body builds them
+__device__ inline void poisson_row(const PoissonTabs& t, long m, double khsq, int k, double& a,
+                                   double& b, double& c) {
+    const int nz = t.nz;
+    if (k == 0) {
+        a = 0.0;
+        b = -(t.cs[0] + khsq);
+        b += t.cl[0];
+        if (nz == 1) b += t.cr[0];
+        c = nz > 1 ? t.cr[0] : 0.0;
+        if (m == 0) {
+            b = 1.0;
+            c = 0.0;
+        }
+        return;
+    }
+    a = t.cl[k];
+    b = -(t.cs[k] + khsq);
+    c = 0.0;
+    if (k == nz - 1) b += t.cr[nz - 1];
+    else c = t.cr[k];
+}
+
+__device__ inline void poisson_spike_body(const PoissonTabs& t, double2* __restrict__ d,
+                                          long unit, double* __restrict__ lds) {
+    const int nz = t.nz;
+    const long M = (long)t.nx * t.nky;
+    const int S = spike_segs(nz), MB = 256 / S;
+    const int R = (nz + S - 1) / S;          // levels per segment (<= kSpikeR)
+    const int nseg = (nz + R - 1) / R;       // segments holding levels
+    const int ml = threadIdx.x % MB, seg = threadIdx.x / MB;
+    const long m = unit * MB + ml;
+    const bool live = m < M && seg < nseg;
+    const int k0 = seg * R;
+    const int nr = live ? min(R, nz - k0) : 0;
+    double khsq = 0.0;
+    if (live) khsq = t.kxsq[m / t.nky] + t.kysq[m % t.nky];
+    double yr[kSpikeR], yi[kSpikeR], al[kSpikeR], be[kSpikeR], cp[kSpikeR];
+    // forward elimination of the segment (couplings to the neighbours moved
+    // to the alpha / beta right-hand sides)
+#pragma unroll
+    for (int i = 0; i < kSpikeR; ++i) {
+        if (i < nr) {
+            const int k = k0 + i;
+            double a, b, c;
+            poisson_row(t, m, khsq, k, a, b, c);
+            const double2 dk = d[(long)k * M + m];
+            const double ra = (i == 0) ? -a : 0.0;           // coupling below
+            const double rb = (i == nr - 1) ? -c : 0.0;      // coupling above
+            if (i == 0) {
+                yr[0] = dk.x / b;
+                yi[0] = dk.y / b;
+                al[0] = ra / b;
+                be[0] = rb / b;
+                cp[0] = (nr > 1) ? c / b : 0.0;
+            } else {
+                const double den = b - a * cp[i - 1];
+                yr[i] = (dk.x - a * yr[i - 1]) / den;
+                yi[i] = (dk.y - a * yi[i - 1]) / den;
+                al[i] = (0.0 - a * al[i - 1]) / den;
+                be[i] = (rb - a * be[i - 1]) / den;
+                cp[i] = (i < nr - 1) ? c / den : 0.0;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = kSpikeR - 2; i >= 0; --i) {
+        if (i < nr - 1) {
+            yr[i] -= cp[i] * yr[i + 1];
+            yi[i] -= cp[i] * yi[i + 1];
+            al[i] -= cp[i] * al[i + 1];
+            be[i] -= cp[i] * be[i + 1];
+        }
+    }
+    // the segment's first and last levels for the reduced system
+    double* slot = lds + (seg * MB + ml) * 8;
+    if (live) {
+        slot[0] = yr[0];
+        slot[1] = yi[0];
+        slot[2] = al[0];
+        slot[3] = be[0];
+        slot[4] = yr[nr - 1];
+        slot[5] = yi[nr - 1];
+        slot[6] = al[nr - 1];
+        slot[7] = be[nr - 1];
+    }
+    __syncthreads();
+    // reduced system, one thread per mode: with L_{s-1} = P + Q F_s,
+    // F_s = U_s + V_s F_{s+1} and L_s = P_s + Q_s F_{s+1}
+    if (seg == 0 && m < M) {
+        double Pr = 0.0, Pi = 0.0, Q = 0.0;
+        for (int s = 0; s < nseg; ++s) {
+            double* q = lds + (s * MB + ml) * 8;
+            const double y0r = q[0], y0i = q[1], a0 = q[2], b0 = q[3];
+            const double y1r = q[4], y1i = q[5], a1 = q[6], b1 = q[7];
+            const double D = 1.0 - a0 * Q;
+            const double Ur = (y0r + a0 * Pr) / D, Ui = (y0i + a0 * Pi) / D, V = b0 / D;
+            const double nPr = y1r + a1 * Pr + a1 * Q * Ur, nPi = y1i + a1 * Pi + a1 * Q * Ui;
+            const double nQ = a1 * Q * V + b1;
+            q[0] = Ur;
+            q[1] = Ui;
+            q[2] = V;
+            q[4] = nPr;
+            q[5] = nPi;
+            q[6] = nQ;
+            Pr = nPr;
+            Pi = nPi;
+            Q = nQ;
+        }
+        double Fr = 0.0, Fi = 0.0;  // F_{s+1}
+        for (int s = nseg - 1; s >= 0; --s) {
+            double* q = lds + (s * MB + ml) * 8;
+            const double Lr = q[4] + q[6] * Fr, Li = q[5] + q[6] * Fi;
+            const double nFr = q[0] + q[2] * Fr, nFi = q[1] + q[2] * Fi;
+            q[0] = nFr;  // F_s
+            q[1] = nFi;
+            q[4] = Lr;   // L_s
+            q[5] = Li;
+            Fr = nFr;
+            Fi = nFi;
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+    double Lr = 0.0, Li = 0.0, Fr = 0.0, Fi = 0.0;
+    if (seg > 0) {
+        const double* q = lds + ((seg - 1) * MB + ml) * 8;
+        Lr = q[4];
+        Li = q[5];
+    }
+    if (seg < nseg - 1) {
+        const double* q = lds + ((seg + 1) * MB + ml) * 8;
+        Fr = q[0];
+        Fi = q[1];
+    }
+#pragma unroll
+    for (int i = 0; i < kSpikeR; ++i) {
+        if (i < nr) {
+            const double xr = yr[i] + al[i] * Lr + be[i] * Fr;
+            const double xi = yi[i] + al[i] * Li + be[i] * Fi;
+            d[(long)(k0 + i) * M + m] = make_double2(xr, xi);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // projection u -= dt grad p from the plane-major (unnormalised) inverse FFT
 // ---------------------------------------------------------------------------
 // Also writes the interior of the ghost-shaped pressure p = P / (nx ny).
