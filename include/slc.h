@@ -254,6 +254,13 @@ int slc_bulk_force(slc_plan* plan, double* u, double* v, double* w, double U_bul
 int slc_bulk_force_noalloc(slc_plan* plan, double* u, double* v, double* w, double U_bulk,
                            double dt, double* out, void* stream);
 
+/* Change the plan's kinematic viscosity nu (= 1/Re_b).  No counterpart in
+ * the reference, where nu is fixed when the solver is built (solver.py:
+ * self.nu = 1/Re from the config): lets a caller step runs of several Re_b
+ * on one plan (a serving loop keeps one solver per grid).  The z-solve
+ * factors, which hold nu, are rebuilt at their next use. */
+int slc_set_viscosity(slc_plan* plan, double nu);
+
 /* Options.
  * SLC_OPT_GATHER_TILE (no counterpart in the reference): -1 = direct
  * global-load kernels; 0 = fused LDS-tiled step kernels (4x4x16 arrival

@@ -23,7 +23,7 @@ EXPORTS = (
     "slc_bulk_velocity", "slc_bulk_force", "slc_bulk_force_noalloc", "slc_diagnostics",
     "slc_init_solenoidal", "slc_init_solenoidal_work",
     "slc_imex_rhs",
-    "slc_ab2_update", "slc_set_option",
+    "slc_ab2_update", "slc_set_option", "slc_set_viscosity",
     "slc_profile_enable", "slc_profile_report",
     "slc_ens_create", "slc_ens_destroy", "slc_ens_member_bytes", "slc_ens_add",
     "slc_ens_remove", "slc_ens_count", "slc_ens_bytes", "slc_ens_trim_pool",
@@ -94,6 +94,7 @@ def load():
         "slc_imex_rhs": ([vp, vp, vp, vp, vp, vp, vp, vp], i),
         "slc_ab2_update": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, d, vp], i),
         "slc_set_option": ([vp, i, i], i),
+        "slc_set_viscosity": ([vp, d], i),
         "slc_profile_enable": ([vp, i], i),
         "slc_profile_report": ([vp, ctypes.c_char_p, ctypes.c_int64], i),
         "slc_ens_create": ([i, i, i, ctypes.POINTER(vp)], i),

@@ -735,6 +735,14 @@ int slc_bulk_force_noalloc(slc_plan* p, double* u, double* v, double* w, double 
     return SLC_OK;
 }
 
+int slc_set_viscosity(slc_plan* p, double nu) {
+    if (int rc = check_plan(p)) return rc;
+    if (!(nu > 0.0) || !std::isfinite(nu)) return fail(SLC_EINVAL, "nu must be positive, got %g", nu);
+    p->g.nu = nu;
+    for (int c = 0; c < 3; ++c) p->zvalid[c] = false;  // the z-solve factors hold nu
+    return SLC_OK;
+}
+
 int slc_set_option(slc_plan* p, int option, int value) {
     if (int rc = check_plan(p)) return rc;
     switch (option) {

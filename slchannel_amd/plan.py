@@ -307,6 +307,11 @@ class HipPlan:
                                             *[_ptr(t) for t in r], *rp, float(dt),
                                             self._stream()), "slc_ab2_update")
 
+    def set_viscosity(self, nu):
+        """slc_set_viscosity: the plan's nu (z-solve factors rebuilt at next use)."""
+        _lib.check(self._lib.slc_set_viscosity(self._h, float(nu)), "slc_set_viscosity")
+        self.nu = float(nu)
+
     def init_solenoidal(self, W, L, ab, zt, phase, urms, work, u, v, w):
         """Seeded solenoidal interiors of u, v, w (slc_init_solenoidal): ab, zt,
         phase and work are float64 device tensors laid out as include/slc.h

@@ -1,26 +1,34 @@
 """Serving loop: continuous batching of independent channel runs on one GPU.
 
 Requests (a channel config, a seed, a number of steps, an arrival time)
-arrive over time.  The loop initialises admitted requests on the device and
-advances every admitted run by one SL-BDF2 step per iteration with the
-batched ensemble (one launch per substep for all runs, include/slc_ens.h).
-Admission needs the ensemble's HBM footprint plus the newcomer's
-(slc_ens_member_bytes) to stay within the budget, and follows the device's
-compute: the oldest waiting request joins when the batch is empty or its
-grid points fit the batch budget (`batch_points`), so runs that each fill
-the GPU take it one after the other at full speed, in arrival order; and any
-arrived small request (at most `small_points` grid points) joins at once
-while the small runs' points fit the budget -- it rides in the launches of
-whatever runs, costing its own arithmetic and no launches of its own.  Runs join and
-leave between steps: a run admitted while others are mid-flight bootstraps
-with BDF1 inside the same launches in which the others take BDF2 steps, and
-a run that has done its steps is copied out and its memory released, so a
-waiting request can take its place.
+arrive over time.  The loop initialises admitted requests on the device
+(slc_init_solenoidal, a few launches per request) and advances the admitted
+runs in two lanes that step concurrently on their own streams:
+
+* the solo lane: one large request (more than `small_points` grid points,
+  one step fills the device) at a time, in arrival order, stepped by the
+  single-run solver (SLChannelFlow, the plain per-run path's kernels, so it
+  runs at the plain path's speed and gives its doubles);
+* the batch: small requests advanced together by the batched ensemble (one
+  launch per substep for all of them, include/slc_ens.h).  The oldest
+  waiting small request joins when the batch is empty or its grid points fit
+  the batch budget (`batch_points`); any arrived small request joins while
+  the small runs' points fit the budget -- it rides in the launches of the
+  batch, costing its own arithmetic and no launches of its own, beside the
+  solo run's steps.
+
+Admission needs the server's HBM footprint (ensemble, initialisers, solo
+solvers) plus the newcomer's (slc_ens_member_bytes) to stay within the
+budget.  Runs join and leave between steps: a run admitted while others are
+mid-flight bootstraps with BDF1 inside the same launches in which the others
+take BDF2 steps, and a run that has done its steps is copied out and its
+memory released, so a waiting request can take its place.
 
 Reported per run: arrival, admission and completion times, every step's
-latency (the device time of the batched step it took part in, from
-hipEvents), its final run diagnostics (max|div|, u_tau, CFL, non-finite
-entries; slc_ens_diagnostics), a status, and optionally its final fields.
+latency (the device time of the step it took part in, from hipEvents), its
+final run diagnostics (max|div|, u_tau, CFL, non-finite entries;
+slc_ens_diagnostics / slc_diagnostics), a status, and optionally its final
+fields.
 Aggregates: p50 / p99 step latency, p50 / p99 time in system, member steps
 per second over the span from the first admission to the last completion.
 
@@ -144,9 +152,42 @@ def validate(req):
     return None
 
 
+# Per grid and device: the single-run solver that steps the server's large
+# requests alone (the plain per-run path), apart from the initialiser of
+# the same grid so that the next request's initial state can be built while
+# a run steps.
+_SOLO_CACHE = {}
+
+
+def _solo_key(config, device):
+    """The solo solver's cache key: the whole config except the bulk Reynolds
+    number and velocity, which are set per run (set_reynolds, U_bulk)."""
+    import json
+
+    cfg = json.loads(json.dumps(config))
+    cfg.get("flow", {}).pop("Re", None)
+    cfg.get("flow", {}).pop("U_bulk", None)
+    return (json.dumps(cfg, sort_keys=True), str(device))
+
+
+def _solo_flow(config, device):
+    """The solo solver for this request's grid and options, set to its Re_b
+    and U_b (built once per key)."""
+    from .solver import SLChannelFlow
+
+    key = _solo_key(config, device)
+    flow = _SOLO_CACHE.get(key)
+    if flow is None:
+        flow = _SOLO_CACHE[key] = SLChannelFlow(config=config, device=device)
+    flow.set_reynolds(float(config["flow"]["Re"]))
+    flow.U_bulk = float(config["flow"].get("U_bulk", 1.0))
+    return flow
+
+
 def init_cache_bytes(device):
-    """Device bytes held by the per-grid initialisers of `device` (their plans
-    and fields), which share the HBM budget with the members."""
+    """Device bytes held by the per-grid initialisers and solo solvers of
+    `device` (their plans, fields and histories), which share the HBM budget
+    with the members."""
     def tensors(obj):
         out = []
         for t in vars(obj).values():
@@ -157,7 +198,7 @@ def init_cache_bytes(device):
         return out
 
     total = 0
-    for key, flow in _INIT_CACHE.items():
+    for key, flow in list(_INIT_CACHE.items()) + list(_SOLO_CACHE.items()):
         if key[-1] != str(device):
             continue
         total += flow.plan.workspace_bytes()
@@ -211,8 +252,10 @@ class Server:
         # admissions (initial fields, dt) run on one stream per grid size, so
         # they overlap the batched steps instead of queueing behind them, and
         # two admissions of one grid (which share its initialiser's plan)
-        # stay ordered
+        # stay ordered; the solo lane steps on a stream of its own (default
+        # priority, as every stream here)
         self._streams = {}
+        self._solo_stream = torch.cuda.Stream(self.device)
 
     def _stream_for(self, config):
         key = _grid_key(config)
@@ -223,7 +266,8 @@ class Server:
 
     def warmup(self, configs):
         """Server start-up for a grid catalogue: builds the per-grid initialiser
-        (its rocFFT plans) of every distinct grid before requests arrive."""
+        (its rocFFT plans) of every distinct grid, and the solo solver of every
+        large one, before requests arrive."""
         seen = set()
         for cfg in configs:
             c = _config_grid(cfg)
@@ -234,6 +278,8 @@ class Server:
             seen.add(key)
             with torch.cuda.stream(self._stream_for(cfg)):
                 initial_fields(cfg, 0, device=self.device)
+            if _points(cfg) > self.small_points:
+                _solo_flow(cfg, self.device)
         torch.cuda.synchronize(self.device)
 
     def footprint(self):
@@ -241,11 +287,16 @@ class Server:
         class buffers) plus the per-grid initialisers."""
         return self.ens.bytes() + init_cache_bytes(self.device)
 
-    def fits(self, req, reserved=0, preparing=0, trim=False):
+    def fits(self, req, reserved=0, preparing=0, trim=False, solo=False):
         """Whether the request fits the budget beside what the server holds.
         Released arenas kept for reuse count as held; with trim=True, the
-        oldest of them are freed when that makes the request fit."""
-        if len(self.ens) + preparing >= self.max_members:
+        oldest of them are freed when that makes the request fit.  A solo run
+        (solo=True) needs nothing more once its grid's solver exists (the
+        solver is counted in the footprint), else about a member's bytes."""
+        if solo:
+            if _solo_key(req.config, self.device) in _SOLO_CACHE:
+                return True
+        elif len(self.ens) + preparing >= self.max_members:
             return False
         need = member_bytes(req.config, self.storage)
         room = self.budget - self.footprint() - reserved
@@ -258,32 +309,40 @@ class Server:
             self.ens.trim_pool(pooled - (need - room))
         return True
 
-    def _pick(self, pending, now, active_pts, small_pts, reserved, preparing):
-        """The next request to admit (its index in pending) or None: the
-        oldest arrived one when the batch is empty or it fits the batch
-        budget, else the oldest arrived small one that fits the small runs'
-        budget.  Both also need the HBM budget."""
+    def _pick(self, pending, now, active_pts, small_pts, reserved, preparing, solo_busy):
+        """The next request to admit and its lane ("solo" or "ens"), or None.
+        The oldest arrived request goes first: a large one (more than
+        `small_points` grid points: one step fills the device) to the solo
+        lane once it is free, a small one into the batch when the batch is
+        empty or it fits the batch budget.  Otherwise the oldest arrived small
+        request that fits the small runs' budget joins the batch.  All need
+        the HBM budget."""
         if not pending or pending[0].arrival > now:
             return None
         head = _points(pending[0].config)
-        if (active_pts == 0 or active_pts + head <= self.batch_points) and \
+        if head > self.small_points:
+            if not solo_busy and self.fits(pending[0], reserved, preparing, trim=True,
+                                           solo=True):
+                return 0, "solo"
+        elif (active_pts == 0 or active_pts + head <= self.batch_points) and \
                 self.fits(pending[0], reserved, preparing, trim=True):
-            return 0
+            return 0, "ens"
         for i, r in enumerate(pending[1:], 1):
             if r.arrival > now:
                 break
             p = _points(r.config)
             if p <= self.small_points and small_pts + p <= self.batch_points and \
                     self.fits(r, reserved, preparing, trim=True):
-                return i
+                return i, "ens"
         return None
 
     def run(self, requests, log=None, inflight=3):
         """Serve `requests` (arrival times relative to the start); returns
         (results by rid, metrics dict).  Steps are pipelined: the host keeps
-        at most `inflight` batched steps queued ahead of the device and only
-        synchronises to retire a finished run; each step's latency is its
-        device time from hipEvents."""
+        at most `inflight` steps of each lane queued ahead of the device and
+        only synchronises to retire a finished run; each step's latency is its
+        device time from hipEvents.  The solo lane (one large run, the
+        single-run solver on its own stream) and the batch run concurrently."""
         results = {}
         pending = []
         for r in sorted(requests, key=lambda r: (r.arrival, r.rid)):
@@ -297,11 +356,13 @@ class Server:
                                         finished=r.arrival, status="rejected_invalid")
                 results[r.rid].diag = [why]
         active = {}  # member id -> (request, result)
-        queued = []  # (start event, end event, results stepped)
+        queued = []  # batch steps: (start event, end event, results stepped)
+        solo = None  # the solo run: dict(req, res, flow, dt)
+        solo_q = []  # solo steps: (start event, end event, result)
         retiring = []  # (event after the copies out, request, result, pinned stats, diag)
-        preparing = []  # (event, request, fields, dt, pinned probe, stream, bytes)
-        guards = {}  # member id -> BlowupGuard
-        checks = []  # (event, member id, rid, pinned diagnostics)
+        preparing = []  # (event, request, fields, dt, pinned probe, stream, bytes, lane)
+        guards = {}  # member id or "solo" -> BlowupGuard
+        checks = []  # (event, member id or "solo", rid, pinned diagnostics)
         t0 = time.perf_counter()
         n_member_steps = 0
         first_admit = None
@@ -310,29 +371,65 @@ class Server:
         step_ms = []
         pts = {}  # member id -> grid points
 
-        def drain(keep):
-            while len(queued) > keep:
-                ev0, ev1, stepped = queued.pop(0)
+        def drain(q, keep):
+            while len(q) > keep:
+                ev0, ev1, stepped = q.pop(0)
                 ev1.synchronize()
                 ms = ev0.elapsed_time(ev1)
                 step_ms.append(ms)
                 for res in stepped:
                     res.latencies_ms.append(ms)
 
-        while pending or active or preparing:
+        def copy_out(res, stats, diag, stream):
+            """Pinned copies of a finished run's stats and diagnostics, read
+            once the returned event has completed."""
+            hs = torch.empty(N_STATS, dtype=torch.float64, pin_memory=True)
+            hd = torch.empty(5, dtype=torch.float64, pin_memory=True)
+            with torch.cuda.stream(stream):
+                hs.copy_(stats, non_blocking=True)
+                hd.copy_(diag, non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record()
+            return ev, hs, hd
+
+        def retire_solo(status):
+            nonlocal solo
+            req, res, flow, dt = solo["req"], solo["res"], solo["flow"], solo["dt"]
+            res.status = status
+            st = self._solo_stream
+            with torch.cuda.stream(st):
+                if self.keep_fields:
+                    res.fields = {k: getattr(flow, k).clone() for k in "uvwp"}
+                diag = torch.empty(5, dtype=torch.float64, device=self.device)
+                flow.plan.diagnostics(flow.u, flow.v, flow.w, flow.p, dt, diag)
+                b = flow._bulk
+                stats = torch.stack([b[2], b[1], flow.sl.n_clamped_last.to(torch.float64),
+                                     torch.full_like(b[0], float(res.steps)), b[3]])
+            ev, hs, hd = copy_out(res, stats, diag, st)
+            retiring.append((ev, req, res, hs, hd))
+            guards.pop("solo", None)
+            solo = None
+
+        def n_running():
+            return len(active) + (1 if solo is not None else 0)
+
+        while pending or active or preparing or solo is not None:
             now = time.perf_counter() - t0
             ta = time.perf_counter()
             # admission, FIFO over the requests that have arrived: the initial
             # state is launched on the grid's stream and read back (dt, the
             # finiteness check) through pinned memory once that stream is done
-            reserved = sum(x[6] for x in preparing)
-            active_pts = sum(pts.values()) + sum(_points(x[1].config) for x in preparing)
+            reserved = sum(x[6] for x in preparing if x[7] == "ens")
+            ens_prep = [x for x in preparing if x[7] == "ens"]
+            active_pts = sum(pts.values()) + sum(_points(x[1].config) for x in ens_prep)
             small_pts = sum(p for p in pts.values() if p <= self.small_points) + sum(
-                _points(x[1].config) for x in preparing
+                _points(x[1].config) for x in ens_prep
                 if _points(x[1].config) <= self.small_points)
-            pick = self._pick(pending, now, active_pts, small_pts, reserved, len(preparing))
+            solo_busy = solo is not None or any(x[7] == "solo" for x in preparing)
+            pick = self._pick(pending, now, active_pts, small_pts, reserved, len(ens_prep),
+                              solo_busy)
             if pick is not None:
-                req = pending.pop(pick)
+                req = pending.pop(pick[0])
                 st = self._stream_for(req.config)
                 with torch.cuda.stream(st):
                     fields, dt, probe = _prepare_async(req, self.device)
@@ -341,9 +438,9 @@ class Server:
                     ev = torch.cuda.Event()
                     ev.record()
                 preparing.append((ev, req, fields, dt, h, st,
-                                  member_bytes(req.config, self.storage)))
+                                  member_bytes(req.config, self.storage), pick[1]))
             while preparing and preparing[0][0].query():
-                _, req, fields, dt, h, st, _ = preparing.pop(0)
+                ev, req, fields, dt, h, st, _, lane = preparing.pop(0)
                 probe = h.tolist()
                 if dt is None:
                     dt = _dt_of(req, probe[0])
@@ -353,22 +450,36 @@ class Server:
                     res.admitted = res.finished = time.perf_counter() - t0
                     results[req.rid] = res
                     continue
-                with torch.cuda.stream(st):
-                    mid = self.ens.add(req.config, dt, fields=fields)
-                res = Result(rid=req.rid, admitted=time.perf_counter() - t0, dt=dt)
+                res = Result(rid=req.rid, dt=dt)
                 res.arrival = req.arrival
-                guards[mid] = BlowupGuard(req.u_tau_nominal, self.blowup_factor)
-                active[mid] = (req, res)
-                pts[mid] = _points(req.config)
+                if lane == "solo":
+                    flow = _solo_flow(req.config, self.device)
+                    sst = self._solo_stream
+                    sst.wait_event(ev)
+                    for t in fields:
+                        if t is not None:
+                            t.record_stream(sst)
+                    with torch.cuda.stream(sst):
+                        flow.set_fields(*fields)
+                    solo = {"req": req, "res": res, "flow": flow, "dt": dt}
+                    guards["solo"] = BlowupGuard(req.u_tau_nominal, self.blowup_factor)
+                else:
+                    with torch.cuda.stream(st):
+                        mid = self.ens.add(req.config, dt, fields=fields)
+                    guards[mid] = BlowupGuard(req.u_tau_nominal, self.blowup_factor)
+                    active[mid] = (req, res)
+                    pts[mid] = _points(req.config)
+                res.admitted = time.perf_counter() - t0
                 first_admit = res.admitted if first_admit is None else first_admit
                 if log:
                     c = _config_grid(req.config)
                     log(f"admit {req.rid} {c['nx']}x{c['ny']}x{c['nz']} "
                         f"steps={req.n_steps} footprint={self.footprint()} "
-                        f"active={len(active)}")
+                        f"active={n_running()} lane={lane}")
             host["admit_s"] += time.perf_counter() - ta
-            if not active and not preparing and pending and pending[0].arrival <= now and \
-                    not self.fits(pending[0]):
+            if not active and solo is None and not preparing and pending and \
+                    pending[0].arrival <= now and not self.fits(
+                        pending[0], solo=_points(pending[0].config) > self.small_points):
                 # nothing runs that could free memory: the server's fixed
                 # footprint (pooled arenas, initialisers) leaves no room
                 req = pending.pop(0)
@@ -377,7 +488,7 @@ class Server:
                 res.diag = ["does not fit the HBM budget beside the server's fixed footprint"]
                 results[req.rid] = res
                 continue
-            if not active:
+            if not active and solo is None:
                 if preparing:
                     time.sleep(1e-4)
                 elif pending:
@@ -386,38 +497,68 @@ class Server:
                         time.sleep(min(wait, 0.05))
                 continue
             tw = time.perf_counter()
-            drain(inflight - 1)
+            if solo is not None:
+                drain(solo_q, inflight - 1)
+            if active:
+                drain(queued, inflight - 1)
             ts = time.perf_counter()
             host["wait_s"] += ts - tw
-            ev0 = torch.cuda.Event(enable_timing=True)
-            ev1 = torch.cuda.Event(enable_timing=True)
-            ev0.record()
-            self.ens.step()
-            ev1.record()
-            host["step_s"] += time.perf_counter() - ts
-            queued.append((ev0, ev1, [res for _, res in active.values()]))
-            n_member_steps += len(active)
             done = []
-            for mid, (req, res) in active.items():
+            if solo is not None:
+                flow, res, req = solo["flow"], solo["res"], solo["req"]
+                with torch.cuda.stream(self._solo_stream):
+                    ev0 = torch.cuda.Event(enable_timing=True)
+                    ev1 = torch.cuda.Event(enable_timing=True)
+                    ev0.record()
+                    flow.step_sl_bdf2(solo["dt"])
+                    ev1.record()
+                solo_q.append((ev0, ev1, [res]))
+                n_member_steps += 1
                 res.steps += 1
                 if res.steps >= req.n_steps:
-                    done.append((mid, "ok"))
+                    done.append(("solo", "ok"))
                 elif self.guard_every and res.steps % self.guard_every == 0:
-                    h = torch.empty(5, dtype=torch.float64, pin_memory=True)
-                    h.copy_(self.ens.diagnostics(mid), non_blocking=True)
-                    ev = torch.cuda.Event()
-                    ev.record()
-                    checks.append((ev, mid, req.rid, h))
+                    with torch.cuda.stream(self._solo_stream):
+                        d = torch.empty(5, dtype=torch.float64, device=self.device)
+                        flow.plan.diagnostics(flow.u, flow.v, flow.w, flow.p, solo["dt"], d)
+                        hg = torch.empty(5, dtype=torch.float64, pin_memory=True)
+                        hg.copy_(d, non_blocking=True)
+                        ev = torch.cuda.Event()
+                        ev.record()
+                    checks.append((ev, "solo", req.rid, hg))
+            if active:
+                ev0 = torch.cuda.Event(enable_timing=True)
+                ev1 = torch.cuda.Event(enable_timing=True)
+                ev0.record()
+                self.ens.step()
+                ev1.record()
+                queued.append((ev0, ev1, [res for _, res in active.values()]))
+                n_member_steps += len(active)
+                for mid, (req, res) in active.items():
+                    res.steps += 1
+                    if res.steps >= req.n_steps:
+                        done.append((mid, "ok"))
+                    elif self.guard_every and res.steps % self.guard_every == 0:
+                        h = torch.empty(5, dtype=torch.float64, pin_memory=True)
+                        h.copy_(self.ens.diagnostics(mid), non_blocking=True)
+                        ev = torch.cuda.Event()
+                        ev.record()
+                        checks.append((ev, mid, req.rid, h))
+            host["step_s"] += time.perf_counter() - ts
             # guard results whose copies have landed (no waiting)
             while checks and checks[0][0].query():
                 _, mid, rid, h = checks.pop(0)
-                if mid in active and active[mid][0].rid == rid and \
-                        all(m != mid for m, _ in done):
+                running = (solo is not None and solo["req"].rid == rid) if mid == "solo" \
+                    else (mid in active and active[mid][0].rid == rid)
+                if running and all(m != mid for m, _ in done):
                     verdict = guards[mid].check(h.tolist())
                     if verdict is not None:
                         done.append((mid, "stopped_" + verdict))
             tr = time.perf_counter()
             for mid, status in done:
+                if mid == "solo":
+                    retire_solo(status)
+                    continue
                 req, res = active.pop(mid)
                 guards.pop(mid, None)
                 pts.pop(mid, None)
@@ -426,12 +567,8 @@ class Server:
                 # the arena is reused only by work queued after them
                 if self.keep_fields:
                     res.fields = self.ens.get_fields(mid)
-                hs = torch.empty(N_STATS, dtype=torch.float64, pin_memory=True)
-                hd = torch.empty(5, dtype=torch.float64, pin_memory=True)
-                hs.copy_(self.ens.stats(mid), non_blocking=True)
-                hd.copy_(self.ens.diagnostics(mid), non_blocking=True)
-                ev = torch.cuda.Event()
-                ev.record()
+                ev, hs, hd = copy_out(res, self.ens.stats(mid), self.ens.diagnostics(mid),
+                                      torch.cuda.current_stream(self.device))
                 retiring.append((ev, req, res, hs, hd))
                 self.ens.remove(mid)
             while retiring and retiring[0][0].query():
@@ -445,7 +582,8 @@ class Server:
             res.finished = time.perf_counter() - t0
             res.stats, res.diag = hs.tolist(), hd.tolist()
             results[req.rid] = res
-        drain(0)
+        drain(queued, 0)
+        drain(solo_q, 0)
         t_end = time.perf_counter() - t0
         lats = np.array([x for r in results.values() for x in r.latencies_ms])
         span = t_end - (first_admit or 0.0)

@@ -200,6 +200,14 @@ class SLChannelFlow:
         """solver.py:599-618: one projection with dt = 1 and a ghost fill."""
         self.plan.project_step(self.u, self.v, self.w, 1.0, 1.0, self.p)
 
+    def set_reynolds(self, Re):
+        """Change the bulk Reynolds number of this solver (nu = 1/Re; the
+        plan's z-solve factors are rebuilt at their next use).  No
+        counterpart in the reference, which fixes nu when the solver is built;
+        with set_fields it lets one solver step runs of several Re_b."""
+        self.nu = 1.0 / float(Re)
+        self.plan.set_viscosity(self.nu)
+
     def apply_bc_uvw(self):
         """apply_bc_all (solver.py:70-107) on the current fields."""
         self.plan.apply_bc(self.u, self.v, self.w)

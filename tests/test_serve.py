@@ -49,7 +49,7 @@ def test_results_same_alone_and_in_the_mix(check):
 
     def note(msg):
         log.append(msg)
-        active_max[0] = max(active_max[0], int(msg.split("active=")[1]))
+        active_max[0] = max(active_max[0], int(msg.split("active=")[1].split()[0]))
         foot_max[0] = max(foot_max[0], int(msg.split("footprint=")[1].split()[0]))
 
     results, metrics = srv.run(reqs, log=note)
@@ -134,10 +134,10 @@ def test_invalid_requests_rejected_alone(check):
 
 
 def test_compute_aware_admission(check):
-    """Runs that fill the device go one after the other in arrival order; a
-    small request that arrives behind a waiting large one rides along with
-    the running large one instead of queueing.  Every result equals its run
-    alone."""
+    """Runs that fill the device go one after the other in arrival order, in
+    the solo lane (the single-run solver); a small request that arrives
+    behind a waiting large one rides in the batch beside the running large
+    one instead of queueing.  Every result equals its run alone."""
     from slchannel_amd.serve import Server, run_alone
 
     big_a = _req(0, 48, 32, 32, 2800.0, 12, 0.0)
@@ -150,13 +150,16 @@ def test_compute_aware_admission(check):
     log = []
     results, _ = srv.run([big_a, big_b, small], log=log.append)
     order = [int(m.split()[1]) for m in log]
-    active_at = {int(m.split()[1]): int(m.split("active=")[1]) for m in log}
+    active_at = {int(m.split()[1]): int(m.split("active=")[1].split()[0]) for m in log}
     check("all served", all(results[i].status == "ok" for i in range(3)),
           f"{[results[i].status for i in range(3)]}")
     check("admitted in the order large, small, large", order == [0, 2, 1], f"{log}")
     # when the second large run is admitted the first has left: at most the
     # rider and the newcomer are active
     check("second large run waits for the first", active_at[1] <= 2, f"{log}")
+    lanes = {int(m.split()[1]): m.split("lane=")[1] for m in log}
+    check("large runs in the solo lane, the small one in the batch",
+          lanes == {0: "solo", 1: "solo", 2: "ens"}, f"{lanes}")
     for r in (big_a, big_b, small):
         alone, _ = run_alone(r)
         same = all(torch.equal(results[r.rid].fields[k], alone.fields[k]) for k in "uvwp")
