@@ -20,6 +20,8 @@ Prints ONE JSON line.  Also reported:
                 too (fp64_check);
   gather_GBps   the isolated order-P gather of u (slc_gather) at 40 B/point
                 (3x8 B feet + 8 B field + 8 B out), feet from the 2dt foot;
+                gather_GBps_by_order: the same field and feet at orders 4
+                and 6 (a second plan of the other order);
   cpu_baseline  the CPU oracle (oracle/: numpy + OpenMP C) timed on this
                 host for ONE BDF2 step of the same 256^3 state.
 """
@@ -172,6 +174,23 @@ def main():
     plan.profile(False)
     g_cnt, g_tot = gp["gather_u"]
     gather_gbps = 40.0 * N_u / (g_tot / g_cnt * 1e-3) / 1e9
+    gather_by_order = {str(args.order): round(gather_gbps, 1)}
+    # the same gather (same field, same feet) by a plan of the other order
+    from slchannel_amd.plan import HipPlan
+
+    other = 4 if args.order == 6 else 6
+    p2 = HipPlan(n, n, nz, flow.Lx, flow.Ly, flow.Lz, flow.z_f, flow.z_c, flow.gamma,
+                 stretching_type=flow.stretching_type, order=other, nu=flow.nu,
+                 device=flow.device)
+    p2.gather("u", flow.u_nm1, xd, yd, zd, out)
+    torch.cuda.synchronize()
+    p2.profile(True)
+    for _ in range(5):
+        p2.gather("u", flow.u_nm1, xd, yd, zd, out)
+    c2, t2 = p2.profile_report()["gather_u"]
+    p2.profile(False)
+    gather_by_order[str(other)] = round(40.0 * N_u / (t2 / c2 * 1e-3) / 1e9, 1)
+    del p2
 
     # correctness guards on the measured run (no host sync inside the loop)
     div = flow.divergence().abs().max().item()
@@ -218,6 +237,7 @@ def main():
                      if args.traj_order == 2 else None},
         "gather_GBps": round(gather_gbps, 1),
         "gather_frac_hbm": round(gather_gbps / PEAK_HBM_GBPS, 4),
+        "gather_GBps_by_order": gather_by_order,
         "max_div": div,
         "u_bulk": ubv,
         "kernels_ms": kernels,

@@ -230,11 +230,16 @@ class SLChannelFlow:
         return self._bulk[2].clone(), self.forcing
 
     def _implicit_and_project(self, dt_z, rhs_scale, dt_proj, theta=1.0, sl=True):
-        """Ghosts, z-solves, ghosts, projection; returns the buffer holding
+        """(Ghosts,) z-solves, ghosts, projection; returns the buffer holding
         phi: self.p, or for SL steps with the 'inc' option a separate phi
         buffer (self.p then ends as the full pressure p^{n+1})."""
         plan = self.plan
-        plan.apply_bc(self.u, self.v, self.w)
+        # the z-solves read ghost values only for the explicit part of a
+        # theta < 1 right-hand side; at theta = 1 the fill the reference makes
+        # here (solver.py:766, 949) is overwritten by the one after the solves
+        # before anything reads it
+        if theta != 1.0:
+            plan.apply_bc(self.u, self.v, self.w)
         plan.zsolve("u", self.u, dt_z, theta)
         plan.zsolve("v", self.v, dt_z, theta)
         plan.zsolve("w", self.w, dt_z, theta)
