@@ -1,5 +1,7 @@
 // slc_step.hip -- LDS-tiled semi-Lagrangian step kernels for gfx950 (one
 // channel per launch); the tile itself is sl_tile_body in slc_tile.h.
+#include <atomic>
+
 #include "slc_tile.h"
 
 namespace slc {
@@ -20,13 +22,19 @@ namespace slc {
 #define SLC_TILE_WAVES 4
 #endif
 // waves per SIMD the near pass and the gather at given feet are built for
-// (order 6; at order 4 their registers fit six waves without spilling, and
-// six are 8% faster than five at 256^3)
+// (order 6).  At order 4 they are built for seven, with a 21 KB box budget so
+// that seven workgroups fit a CU's 160 KB of LDS (72 VGPRs, 16 B spilled per
+// lane; measured at 256^3: six waves 0.446 ms, seven 0.415 ms, eight with a
+// 19 KB budget spill 40 B per lane and take 0.445 ms).  The order-4 box of a
+// 4x4x16 tile at the usual CFL is about 10 KB.
 #ifndef SLC_TILE_WAVES_GATHER
 #define SLC_TILE_WAVES_GATHER SLC_TILE_WAVES
 #endif
 #ifndef SLC_TILE_WAVES_GATHER4
-#define SLC_TILE_WAVES_GATHER4 6
+#define SLC_TILE_WAVES_GATHER4 7
+#endif
+#ifndef SLC_GTILE_LDS4
+#define SLC_GTILE_LDS4 21504
 #endif
 // waves per SIMD the fused far + near pass is built for
 #ifndef SLC_TILE_WAVES_BDF2
@@ -56,26 +64,120 @@ __global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
 }
 
 // The gather at given feet for the tiles a first pass deferred (their boxes
-// exceed its LDS budget; flags[tile] = 1); the blocks of the other tiles exit.
-// Same tile order as the first pass (xcd_block).  A tile that does not fit
-// this budget either takes the direct-load path.  gfx950 gives one workgroup
-// up to 160 KB of LDS; 76 KB keeps two of these blocks per CU and holds the
-// box of a 4x4x16 tile whose feet spread over +-5 cells in x and y at order 6
-// (at most 20 x 20 x 21 doubles and the row bases).
-constexpr int kWideLds = 77824;
+// exceed its LDS budget; flags[tile] = 1).  Its own tiles are 8 x 8 x 4: for
+// feet spread far from their arrival points in x and y (the +-5-cell random
+// displacement of the 1024x1024x512 benchmark) a flat tile's box is smaller
+// per point than a 4 x 4 x 16 tile's (23 x 23 x 9 against 20 x 20 x 21
+// doubles at order 6), so a 40 KB budget holds it and four workgroups share a
+// CU.  A wide tile runs when a first-pass tile it overlaps is flagged, and
+// gathers only the points of flagged tiles (each point is gathered and its
+// clamp counted once over the two passes).  A box that does not fit this
+// budget either takes the direct-load path.  Measured at 512^3, order 6,
+// +-5 cells: 4 x 4 x 16 wide tiles with 76 KB (two per CU) 16.9 ms, these
+// 11.7 ms with one workgroup per tile and 12.6 ms persistent (below); every
+// tile 8 x 8 x 4 in the first pass 10.3 ms, but the normal gather 16% slower.
+constexpr int kWideTI = 8, kWideTJ = 8, kWideTK = 4;
+constexpr int kWideLds = 40832;  // 160 KB / 4, less the block's other LDS
+#ifndef SLC_WIDE_WAVES
+#define SLC_WIDE_WAVES 4
+#endif
+constexpr int kWideBlocksPerCU = SLC_WIDE_WAVES;
 
+// Whether wide tile wt overlaps a flagged first-pass tile (FI x FJ x FK
+// tiles, numbered like the wide ones, z fastest).  Independent loads; indices
+// past the domain's last tile repeat that tile.
+template <int C>
+__device__ inline bool wide_wanted(const Grid& g, const unsigned* __restrict__ flags, long wt) {
+    constexpr int FI = SLC_GTILE_TI, FJ = SLC_GTILE_TJ, FK = SLC_GTILE_TK;
+    static_assert(kWideTI % FI == 0 && kWideTJ % FJ == 0 && FK % kWideTK == 0,
+                  "a wide tile covers whole first-pass tiles in x, y and lies in one in z");
+    constexpr int NI = kWideTI / FI, NJ = kWideTJ / FJ;
+    const int nzi = Comp<C>::NZI(g);
+    const int nwk = (nzi + kWideTK - 1) / kWideTK, nwj = (g.ny + kWideTJ - 1) / kWideTJ;
+    const int wk = (int)(wt % nwk);
+    const long r = wt / nwk;
+    const int wj = (int)(r % nwj), wi = (int)(r / nwj);
+    const int i0 = wi * kWideTI, j0 = wj * kWideTJ;
+    const int i1 = min(i0 + kWideTI, g.nx) - 1, j1 = min(j0 + kWideTJ, g.ny) - 1;
+    const int ftk = (nzi + FK - 1) / FK, ftj = (g.ny + FJ - 1) / FJ;
+    unsigned anyf = 0u;
+#pragma unroll
+    for (int a = 0; a < NI; ++a)
+#pragma unroll
+        for (int b = 0; b < NJ; ++b) {
+            const int fi = min(i0 / FI + a, i1 / FI), fj = min(j0 / FJ + b, j1 / FJ);
+            anyf |= flags[((long)fi * ftj + fj) * ftk + (wk * kWideTK) / FK];
+        }
+    return anyf != 0u;
+}
+
+// Persistent: a few workgroups per CU walk the wide tiles in chunks of kSB
+// (each lane tests one tile, one round of loads per chunk) and gather the
+// wanted tiles of a chunk one after the other, so that a gather with
+// nothing deferred costs a few hundred short workgroups instead of one per
+// tile: at 256^3 the one-per-tile launch spent 60-100 us dispatching
+// workgroups that exit (order-6 gather 0.84 -> 0.79 ms, order 4 0.42 ->
+// 0.36 ms with this one).  The loop keeps the kernel arguments live, so at
+// order 6 the SGPRs overflow into VGPR lanes and 96 B per lane spill (+-5
+// cells at 512^3: 12.6 ms against 11.7 ms one workgroup per tile; three
+// workgroups per CU, without the spill, 14.2 ms).
 template <int C, int P>
-__global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(2))) void gather_wide_kernel(
+__global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(SLC_WIDE_WAVES))) void gather_wide_kernel(
     Grid g, ArrivalTabs A, const double* __restrict__ F, Feet ft, double* __restrict__ out,
-    unsigned long long* nclamp, const unsigned* __restrict__ flags) {
+    unsigned long long* nclamp, const unsigned* __restrict__ flags, long wtiles) {
     __shared__ double lds[kWideLds / sizeof(double)];
     __shared__ int sh[6];
     __shared__ int s_cnt;
-    const long tile = xcd_block(gridDim.x);
-    if (flags[tile] == 0u) return;
+    __shared__ unsigned long long s_want[kSB / 64];
     const TrajVel V{{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
-    sl_tile_body<double, C, P, MODE_GATHER, SLC_GTILE_TI, SLC_GTILE_TJ, SLC_GTILE_TK, kWideLds>(
-        g, A, V, F, F, F, F, 0.0, out, ft, nclamp, tile, lds, sh, &s_cnt);
+    Redo redo;
+    redo.flags = flags;
+    redo.ti = SLC_GTILE_TI;
+    redo.tj = SLC_GTILE_TJ;
+    redo.tk = SLC_GTILE_TK;
+    const long nchunks = (wtiles + kSB - 1) / kSB;
+    for (long ch = xcd_block(gridDim.x); ch < nchunks; ch += gridDim.x) {
+        const long wt = ch * kSB + threadIdx.x;
+        const bool want = wt < wtiles && wide_wanted<C>(g, flags, wt);
+        const unsigned long long m = __ballot(want);
+        if ((threadIdx.x & 63) == 0) s_want[threadIdx.x >> 6] = m;
+        __syncthreads();
+        for (int w = 0; w < kSB / 64; ++w) {
+            // uniform over the block: scalar loop control around the barriers
+            // of the tile body
+            const unsigned long long mw = s_want[w];
+            unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)mw);
+            unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(mw >> 32));
+            while (lo | hi) {
+                int bit;
+                if (lo) {
+                    bit = __builtin_ctz(lo);
+                    lo &= lo - 1;
+                } else {
+                    bit = 32 + __builtin_ctz(hi);
+                    hi &= hi - 1;
+                }
+                sl_tile_body<double, C, P, MODE_GATHER, kWideTI, kWideTJ, kWideTK, kWideLds>(
+                    g, A, V, F, F, F, F, 0.0, out, ft, nclamp, ch * kSB + w * 64 + bit, lds, sh,
+                    &s_cnt, nullptr, nullptr, redo);
+            }
+        }
+        __syncthreads();  // s_want is rewritten by the next chunk
+    }
+}
+
+// Compute units of the current device (cached per device).
+static int device_cus() {
+    static std::atomic<int> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int n = cache[dev].load(std::memory_order_relaxed);
+    if (n > 0) return n;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        n <= 0)
+        n = 256;
+    cache[dev].store(n, std::memory_order_relaxed);
+    return n;
 }
 
 // Arrival tables of component C: x/dx, y/dy of the arrival columns and the
@@ -210,7 +312,8 @@ static void launch_mode(const Grid& g, const ArrivalTabs& A, const TrajVel& V,
                         const StepArgs& a, hipStream_t s) {
     constexpr bool G = MODE == MODE_NEAR || MODE == MODE_GATHER;
     constexpr int TI = G ? SLC_GTILE_TI : kTI, TJ = G ? SLC_GTILE_TJ : kTJ;
-    constexpr int TK = G ? SLC_GTILE_TK : kTK, LDS = G ? SLC_GTILE_LDS : kLDS;
+    constexpr int TK = G ? SLC_GTILE_TK : kTK;
+    constexpr int LDS = G ? (P == 4 ? SLC_GTILE_LDS4 : SLC_GTILE_LDS) : kLDS;
     constexpr int W = G ? (P == 4 ? SLC_TILE_WAVES_GATHER4 : SLC_TILE_WAVES_GATHER)
                         : (MODE == MODE_BDF2 ? SLC_TILE_WAVES_BDF2 : SLC_TILE_WAVES);
     const long nzi = (C == 2) ? g.nz - 1 : g.nz;
@@ -222,9 +325,16 @@ static void launch_mode(const Grid& g, const ArrivalTabs& A, const TrajVel& V,
                        dim3(kSB), 0, s, g, A, V, a.F, a.Fm, (const T*)a.Sn, (const T*)a.Sm, a.dt,
                        a.out, a.ft, a.ncl, defer);
     if constexpr (MODE == MODE_GATHER && sizeof(T) == 8) {
-        if (defer)
-            hipLaunchKernelGGL((gather_wide_kernel<C, P>), dim3(tiles), dim3(kSB), 0, s, g, A,
-                               (const double*)a.Sn, a.ft, a.out, a.ncl, defer);
+        if (defer) {
+            const long wtiles = ((long)(g.nx + kWideTI - 1) / kWideTI) *
+                                ((g.ny + kWideTJ - 1) / kWideTJ) *
+                                ((nzi + kWideTK - 1) / kWideTK);
+            const long nchunks = (wtiles + kSB - 1) / kSB;
+            const long cap = (long)device_cus() * kWideBlocksPerCU;
+            hipLaunchKernelGGL((gather_wide_kernel<C, P>),
+                               dim3((unsigned)(nchunks < cap ? nchunks : cap)), dim3(kSB), 0, s,
+                               g, A, (const double*)a.Sn, a.ft, a.out, a.ncl, defer, wtiles);
+        }
     }
 }
 

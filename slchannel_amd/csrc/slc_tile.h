@@ -422,13 +422,27 @@ __device__ inline void trace_foot(const Grid& g, const TrajA& V, double xa, doub
 // defer (MODE_GATHER): when non-null, one flag per tile: 1 for a tile whose
 // box does not fit the LDS budget (it is not gathered here but left for a
 // pass with a larger budget), 0 for the others.
+// redo (MODE_GATHER): when redo.flags is non-null, only the points whose tile
+// of an earlier pass (redo.ti x redo.tj x redo.tk tiles, same numbering) has
+// its defer flag set are gathered; the others were done by that pass.
+struct Redo {
+    const unsigned* flags = nullptr;
+    int ti = 1, tj = 1, tk = 1;
+};
+
+__device__ inline bool redo_point(const Redo& r, const Grid& g, int nzi, int i, int j, int k) {
+    if (!r.flags) return true;
+    const int rtk = (nzi + r.tk - 1) / r.tk, rtj = (g.ny + r.tj - 1) / r.tj;
+    return r.flags[((long)(i / r.ti) * rtj + j / r.tj) * rtk + k / r.tk] != 0u;
+}
+
 template <typename T, int C, int P, int MODE, int TI, int TJ, int TK, int LDS_BYTES>
 __device__ __forceinline__ void sl_tile_body(
     const Grid& g, const ArrivalTabs& A, const TrajVel& V, const double* __restrict__ F,
     const double* __restrict__ Fm, const T* __restrict__ Sn, const T* __restrict__ Sm,
     double dt, double* __restrict__ out, const Feet& ft, unsigned long long* nclamp, long tile,
     T* __restrict__ lds, int* sh, int* s_cnt, FootSt* nst = nullptr,
-    unsigned* defer = nullptr) {
+    unsigned* defer = nullptr, Redo redo = Redo{}) {
     constexpr int NPT = TI * TJ * TK;
     static_assert(NPT % kSB == 0, "tile must be a multiple of the block");
     constexpr int PT = NPT / kSB;
@@ -454,7 +468,8 @@ __device__ __forceinline__ void sl_tile_body(
         const int jj = t2 % TJ, ii = t2 / TJ;
         const int i = i0 + ii, j = j0 + jj, k = k0t + kk;
         idx[p] = -1;
-        if (i < g.nx && j < g.ny && k < nzi) {
+        if (i < g.nx && j < g.ny && k < nzi &&
+            (MODE != MODE_GATHER || redo_point(redo, g, nzi, i, j, k))) {
             const long n = ((long)i * g.ny + j) * nzi + k;
             idx[p] = n;
             double xa, ya, za;
