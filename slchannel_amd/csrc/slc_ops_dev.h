@@ -300,16 +300,38 @@ __device__ inline void zsolve_explicit(const Grid& g, double* p, int j0, int n, 
     }
 }
 
+// Orders the wavefront's LDS accesses (a wavefront executes them in order;
+// this keeps the compiler from moving them across).
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS doubles per column of the transposed z-solve (zsolve_scan_cols): one
+// pad after every CH unknowns, so that lane-contiguous reads of CH values
+// spread over the banks.
+template <int CH>
+__host__ __device__ constexpr int zscan_lds(int K = 1) {
+    return 64 * (CH + 1) * K;
+}
+
 // K columns col0 .. col0 + K - 1 (those below ncols) per wavefront: the
 // loads of all K are issued before any is solved (more bytes in flight per
 // wavefront; the factor tables are shared), then each column is solved
-// exactly as a column alone.
+// exactly as a column alone.  With xlds (zscan_lds<CH>(K) doubles of the
+// wavefront's own LDS) the column moves between HBM and the registers
+// through LDS: wavefront-contiguous loads and stores (each load instruction
+// touches 512 consecutive bytes) instead of each lane reading its CH
+// consecutive unknowns (an instruction touches every cache line of the
+// column).  Same unknowns in the same registers, so the same results.
 template <int C, int CH, int K>
 __device__ inline void zsolve_scan_cols(const Grid& g, double* __restrict__ F,
                                         const double* __restrict__ tab, long col0, long ncols,
                                         double beta = 0.0,
                                         const double* __restrict__ dz_c = nullptr,
-                                        const double* __restrict__ dz_f = nullptr) {
+                                        const double* __restrict__ dz_f = nullptr,
+                                        double* __restrict__ xlds = nullptr) {
     const int lane = threadIdx.x & 63;
     const int n = (C == 2) ? g.nz - 1 : g.nz;
     const double* A = tab;
@@ -326,8 +348,26 @@ __device__ inline void zsolve_scan_cols(const Grid& g, double* __restrict__ F,
         const unsigned ug = (unsigned)(ok[kc] ? col : col0);
         const int I = 1 + (int)(ug / (unsigned)g.ny), J = 1 + (int)(ug % (unsigned)g.ny);
         Fc[kc] = F + ((long)I * Comp<C>::SY(g) + J) * Comp<C>::SZ(g) + 1;
+        if (xlds) {
+            double* X = xlds + kc * zscan_lds<CH>();
 #pragma unroll
-        for (int i = 0; i < CH; ++i) p[kc][i] = (ok[kc] && j0 + i <= n) ? Fc[kc][j0 + i] : 0.0;
+            for (int m = 0; m < CH; ++m) {
+                const int e = 64 * m + lane;
+                X[e + e / CH] = (ok[kc] && e <= n) ? Fc[kc][e] : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < CH; ++i)
+                p[kc][i] = (ok[kc] && j0 + i <= n) ? Fc[kc][j0 + i] : 0.0;
+        }
+    }
+    if (xlds) {
+        wave_sync();
+#pragma unroll
+        for (int kc = 0; kc < K; ++kc)
+#pragma unroll
+            for (int i = 0; i < CH; ++i) p[kc][i] = xlds[kc * zscan_lds<CH>() + lane * (CH + 1) + i];
+        wave_sync();
     }
     if (beta != 0.0) {
 #pragma unroll
@@ -415,10 +455,29 @@ __device__ inline void zsolve_scan_cols(const Grid& g, double* __restrict__ F,
     for (int kc = 0; kc < K; ++kc) {
         double z = __shfl_down(R[kc], 1);
         if (lane == 63) z = 0.0;
+        if (xlds) {
+            double* X = xlds + kc * zscan_lds<CH>();
+#pragma unroll
+            for (int i = 0; i < CH; ++i) X[lane * (CH + 1) + i] = p[kc][i] + q[kc][i] * z;
+            continue;
+        }
         if (!ok[kc]) continue;
 #pragma unroll
         for (int i = 0; i < CH; ++i)
             if (j0 + i < n) Fc[kc][j0 + i] = p[kc][i] + q[kc][i] * z;
+    }
+    if (xlds) {
+        wave_sync();
+#pragma unroll
+        for (int kc = 0; kc < K; ++kc) {
+            if (!ok[kc]) continue;
+            const double* X = xlds + kc * zscan_lds<CH>();
+#pragma unroll
+            for (int m = 0; m < CH; ++m) {
+                const int e = 64 * m + lane;
+                if (e < n) Fc[kc][e] = X[e + e / CH];
+            }
+        }
     }
 }
 

@@ -200,6 +200,35 @@ def test_solves_vs_reference(check, golden, tag):
     check("poisson p vs reference", e < 1e-12, f"rel={e:.1e}")
 
 
+@pytest.mark.parametrize("nz", [60, 120, 250, 500, 1000])
+def test_zsolve_column_lengths(check, nz):
+    """z-solves whose columns hold 1, 2, 4, 8 and 16 unknowns per lane (the
+    register-scan kernel's variants) against the oracle's PCR solve
+    (operators.py:382-630, tridiag.py:36-61), theta = 1 and 1/2, both top-wall
+    conditions: within 1e-12 of max|u| (Thomas against PCR rounding)."""
+    from slchannel_amd.grid import generate_grid
+
+    nx, ny, gamma, Lz, nu, dt = 3, 5, 1.8, 2.0, 1.0 / 2000.0, 0.05
+    z_f, z_c, dz_f, dz_c = (t.numpy() for t in generate_grid(gamma, nz, Lz))
+    rng = np.random.default_rng(nz)
+    for top in ("dirichlet", "neumann"):
+        gg = dict(nx=nx, ny=ny, nz=nz, Lx=4 * math.pi, Ly=2 * math.pi, Lz=Lz, gamma=gamma,
+                  z_f=z_f, z_c=z_c, stretching="symmetric")
+        plan = make_plan(gg, order=4, nu=nu, top=top)
+        for theta in (1.0, 0.5):
+            for comp in "uvw":
+                a = rng.normal(size=plan.shapes[comp])
+                f = cuda(a)
+                plan.zsolve(comp, f, dt, theta)
+                if comp == "w":
+                    ref = orc.solve_implicit_diffusion_w(a, dt, nx, ny, nz, dz_c, dz_f, nu, theta)
+                else:
+                    ref = orc.solve_implicit_diffusion_u(a, dt, nx, ny, nz, dz_c, dz_f, nu, theta,
+                                                         top)
+                e = rel_err(f, ref)
+                check(f"nz={nz} {top} theta={theta} {comp}", e < 1e-12, f"rel={e:.1e}")
+
+
 def test_divergence_projection_bulk_vs_oracle(check):
     c = _random_case(24, 20, 28, 4, "symmetric", seed=9)
     plan = make_plan(c)
