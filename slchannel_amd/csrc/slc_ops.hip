@@ -312,20 +312,35 @@ __global__ __launch_bounds__(256) void poisson_thomas_kernel(PoissonTabs t,
     poisson_thomas_body(t, d, cpw, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-__global__ __launch_bounds__(256) void poisson_spike_kernel(PoissonTabs t,
-                                                            double2* __restrict__ d) {
-    __shared__ double lds[256 * 8];
-    poisson_spike_body(t, d, (long)blockIdx.x, lds);
+template <int NT>
+__global__ __launch_bounds__(NT) void poisson_spike_kernel(PoissonTabs t,
+                                                           double2* __restrict__ d) {
+    __shared__ double lds[NT * 8];
+    poisson_spike_body<NT>(t, d, (long)blockIdx.x, lds);
 }
+
+// Segments from which the partition solve runs 512-thread workgroups: at 64
+// or more segments a 256-thread workgroup holds 4 modes or fewer, i.e. plane
+// rows of 64 bytes or less.
+#ifndef SLC_SPIKE_WIDE_SEGS
+#define SLC_SPIKE_WIDE_SEGS 64
+#endif
 
 // The partition solve (32 B per unknown) where nz allows it, else the
 // sequential sweep with c' through the cp scratch.
 hipError_t run_poisson_thomas(const PoissonTabs& t, double* dhat, double* cp, hipStream_t s) {
     long M = (long)t.nx * t.nky;
     if (spike_ok(t.nz)) {
-        const long MB = spike_modes(t.nz);
-        hipLaunchKernelGGL(poisson_spike_kernel, dim3((unsigned)((M + MB - 1) / MB)), dim3(256),
-                           0, s, t, reinterpret_cast<double2*>(dhat));
+        double2* d = reinterpret_cast<double2*>(dhat);
+        if (spike_segs(t.nz) >= SLC_SPIKE_WIDE_SEGS) {
+            const long MB = spike_modes(t.nz, 512);
+            hipLaunchKernelGGL(poisson_spike_kernel<512>, dim3((unsigned)((M + MB - 1) / MB)),
+                               dim3(512), 0, s, t, d);
+        } else {
+            const long MB = spike_modes(t.nz);
+            hipLaunchKernelGGL(poisson_spike_kernel<256>, dim3((unsigned)((M + MB - 1) / MB)),
+                               dim3(256), 0, s, t, d);
+        }
         return hipGetLastError();
     }
     hipLaunchKernelGGL(poisson_thomas_kernel, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s,

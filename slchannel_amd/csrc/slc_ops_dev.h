@@ -648,9 +648,13 @@ __host__ __device__ inline int spike_segs(int nz) {
     return s;
 }
 
-__host__ __device__ inline int spike_modes(int nz) { return 256 / spike_segs(nz); }
+// modes per workgroup of NT threads
+__host__ __device__ inline int spike_modes(int nz, int NT = 256) { return NT / spike_segs(nz); }
 
-__host__ __device__ inline bool spike_ok(int nz) { return nz <= kSpikeR * 256; }
+#ifndef SLC_SPIKE_MAX_NZ
+#define SLC_SPIKE_MAX_NZ (kSpikeR * 256)
+#endif
+__host__ __device__ inline bool spike_ok(int nz) { return nz <= SLC_SPIKE_MAX_NZ; }
 
 // row k of the Poisson system of mode m (khsq = kx^2 + ky^2): a (below),
 // b (diagonal), c (above), as poisson_thomas_body builds them
@@ -676,11 +680,13 @@ __device__ inline void poisson_row(const PoissonTabs& t, long m, double khsq, in
     else c = t.cr[k];
 }
 
+// NT threads per workgroup (lds: NT * 8 doubles).
+template <int NT = 256>
 __device__ inline void poisson_spike_body(const PoissonTabs& t, double2* __restrict__ d,
                                           long unit, double* __restrict__ lds) {
     const int nz = t.nz;
     const long M = (long)t.nx * t.nky;
-    const int S = spike_segs(nz), MB = 256 / S;
+    const int S = spike_segs(nz), MB = NT / S;
     const int R = (nz + S - 1) / S;          // levels per segment (<= kSpikeR)
     const int nseg = (nz + R - 1) / R;       // segments holding levels
     const int ml = threadIdx.x % MB, seg = threadIdx.x / MB;
