@@ -135,7 +135,7 @@ void run_ens_tile_f4(const EnsLaunch& L, int comp, int pass, int shape, unsigned
 void run_ens_tile_f6(const EnsLaunch& L, int comp, int pass, int shape, unsigned grid,
                      hipStream_t s);
 hipError_t run_ens_zsolve(const EnsLaunch& L, int comp, size_t lds_bytes, unsigned grid,
-                          hipStream_t s);
+                          bool short_cols, hipStream_t s);
 hipError_t run_ens_div_planes(const EnsLaunch& L, unsigned grid, hipStream_t s);
 hipError_t run_ens_thomas(const EnsLaunch& L, unsigned grid, hipStream_t s);
 hipError_t run_ens_project(const EnsLaunch& L, unsigned grid, hipStream_t s);

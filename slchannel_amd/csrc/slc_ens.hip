@@ -115,6 +115,7 @@ struct slc_ens {
     std::vector<unsigned> h_start;
     unsigned totals[EK_COUNT] = {};
     size_t zs_lds[3] = {0, 0, 0};
+    bool zs_short[3] = {false, false, false};  // every member: <= 2 unknowns per lane
     // rocFFT
     rocfft_execution_info info = nullptr;
     void* fft_work = nullptr;
@@ -307,7 +308,10 @@ int rebuild(slc_ens* e) {
     std::vector<EnsMember> tab(n);
     e->h_start.assign((size_t)EK_COUNT * (n + 1), 0u);
     std::vector<int> umap[3];
-    for (int k = 0; k < 3; ++k) e->zs_lds[k] = 0;
+    for (int k = 0; k < 3; ++k) {
+        e->zs_lds[k] = 0;
+        e->zs_short[k] = true;
+    }
     for (int i = 0; i < n; ++i) {
         Member* m = e->members[i];
         slc_plan* pl = m->plan;
@@ -346,7 +350,8 @@ int rebuild(slc_ens* e) {
         M.npart = pl->max_partials;
         for (int k = 0; k < 3; ++k) {
             M.zch[k] = zsolve_scan_ch(k, m->nz);
-            M.zcols[k] = M.zch[k] > 0 ? kZScanCols : zsolve_cols(k, m->nz);
+            M.zcols[k] = M.zch[k] > 0 ? kZScanCols * zscan_k(M.zch[k]) : zsolve_cols(k, m->nz);
+            e->zs_short[k] = e->zs_short[k] && (M.zch[k] == 1 || M.zch[k] == 2);
         }
         for (int k = 0; k < 4; ++k) M.nghost[k] = ghost_count(k, m->nx, m->ny, m->nz);
         // work units
@@ -474,7 +479,8 @@ int enqueue_step(slc_ens* e) {
     if (int rc = sl_passes(e)) return rc;
     HIP_OK(run_ens_bc3(launch(e, EK_BC3), 1, pgrid(e, EK_BC3), s));
     for (int c = 0; c < 3; ++c)
-        HIP_OK(run_ens_zsolve(launch(e, EK_ZS0 + c), c, e->zs_lds[c], pgrid(e, EK_ZS0 + c), s));
+        HIP_OK(run_ens_zsolve(launch(e, EK_ZS0 + c), c, e->zs_lds[c], pgrid(e, EK_ZS0 + c),
+                              e->zs_short[c], s));
     HIP_OK(run_ens_bc3(launch(e, EK_BC3), 1, pgrid(e, EK_BC3), s));
     HIP_OK(run_ens_div_planes(launch(e, EK_PLANE), pgrid(e, EK_PLANE), s));
     if (int rc = fft_all(e, true)) return rc;
@@ -949,7 +955,7 @@ static int autotune_sweeps(slc_ens* e, int reps, hipEvent_t t0, hipEvent_t t1,
             HIP_OK(run_ens_ustar(launch(e, EK_ELEM), pgrid(e, EK_ELEM), s));
             for (int c = 0; c < 3; ++c)
                 HIP_OK(run_ens_zsolve(launch(e, EK_ZS0 + c), c, e->zs_lds[c],
-                                      pgrid(e, EK_ZS0 + c), s));
+                                      pgrid(e, EK_ZS0 + c), e->zs_short[c], s));
             HIP_OK(run_ens_div_planes(launch(e, EK_PLANE), pgrid(e, EK_PLANE), s));
             HIP_OK(run_ens_thomas(launch(e, EK_THOMAS), pgrid(e, EK_THOMAS), s));
             return SLC_OK;

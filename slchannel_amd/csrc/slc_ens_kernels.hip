@@ -136,9 +136,10 @@ __global__ __launch_bounds__(256) void ens_zsolve_kernel(ENS_PARAMS) {
         const int nz = M.g.nz;
         const double* tab = M.ztab + (ens_bdf2(st) ? 9 : 0) * (long)(nz + 2) + C * 3L * (nz + 2);
         if (M.zch[C] > 0) {
-            const long col = (long)u * kZScanCols + (threadIdx.x >> 6);
-            if (col < (long)M.g.nx * M.g.ny)
-                zsolve_scan_any<C>(M.g, M.fld[ens_c(st)][C], tab, col, M.zch[C]);
+            const long ncols = (long)M.g.nx * M.g.ny;
+            const long col0 = ((long)u * kZScanCols + (threadIdx.x >> 6)) * zscan_k(M.zch[C]);
+            if (col0 < ncols)
+                zsolve_scan_any<C>(M.g, M.fld[ens_c(st)][C], tab, col0, ncols, M.zch[C]);
         } else {
             zsolve_body<C>(M.g, M.fld[ens_c(st)][C], tab, 0.0, M.dzc, M.dzf, M.zcols[C], u, lds,
                            cbase);
@@ -146,15 +147,47 @@ __global__ __launch_bounds__(256) void ens_zsolve_kernel(ENS_PARAMS) {
     });
 }
 
+// The same for ensembles whose every member's columns hold at most 2
+// unknowns per lane: only those variants are compiled in, so the kernel is
+// built for more waves per SIMD than the general one (which the longest
+// variants and the LDS column path hold at 125 VGPRs, four waves).
+#ifndef SLC_ENS_ZS_SHORT_WAVES
+#define SLC_ENS_ZS_SHORT_WAVES 6
+#endif
+template <int C>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SLC_ENS_ZS_SHORT_WAVES)))
+void ens_zsolve_short_kernel(ENS_PARAMS) {
+    static_assert(zscan_k(1) == zscan_k(2), "one column count for both short variants");
+    ens_for(start, nmem, total, [&](int m, unsigned u) {
+        const EnsMember& M = mem[m];
+        const int st = *M.state;
+        const int nz = M.g.nz;
+        const double* tab = M.ztab + (ens_bdf2(st) ? 9 : 0) * (long)(nz + 2) + C * 3L * (nz + 2);
+        const long ncols = (long)M.g.nx * M.g.ny;
+        const long col0 = ((long)u * kZScanCols + (threadIdx.x >> 6)) * zscan_k(1);
+        double* F = M.fld[ens_c(st)][C];
+        if (col0 < ncols) {
+            if (M.zch[C] == 1) zsolve_scan_cols<C, 1, zscan_k(1)>(M.g, F, tab, col0, ncols);
+            else zsolve_scan_cols<C, 2, zscan_k(2)>(M.g, F, tab, col0, ncols);
+        }
+    });
+}
+
 hipError_t run_ens_zsolve(const EnsLaunch& L, int comp, size_t lds_bytes, unsigned grid,
-                          hipStream_t s) {
+                          bool short_cols, hipStream_t s) {
     if (L.total == 0) return hipSuccess;
-    if (comp == 0)
-        hipLaunchKernelGGL(ens_zsolve_kernel<0>, dim3(grid), dim3(256), lds_bytes, s, ENS_ARGS(L));
-    else if (comp == 1)
-        hipLaunchKernelGGL(ens_zsolve_kernel<1>, dim3(grid), dim3(256), lds_bytes, s, ENS_ARGS(L));
-    else
-        hipLaunchKernelGGL(ens_zsolve_kernel<2>, dim3(grid), dim3(256), lds_bytes, s, ENS_ARGS(L));
+#define SLC_ENS_ZS(K, CV) \
+    hipLaunchKernelGGL((K<CV>), dim3(grid), dim3(256), lds_bytes, s, ENS_ARGS(L))
+    if (short_cols) {
+        if (comp == 0) SLC_ENS_ZS(ens_zsolve_short_kernel, 0);
+        else if (comp == 1) SLC_ENS_ZS(ens_zsolve_short_kernel, 1);
+        else SLC_ENS_ZS(ens_zsolve_short_kernel, 2);
+    } else {
+        if (comp == 0) SLC_ENS_ZS(ens_zsolve_kernel, 0);
+        else if (comp == 1) SLC_ENS_ZS(ens_zsolve_kernel, 1);
+        else SLC_ENS_ZS(ens_zsolve_kernel, 2);
+    }
+#undef SLC_ENS_ZS
     return hipGetLastError();
 }
 

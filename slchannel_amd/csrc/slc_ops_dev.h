@@ -246,6 +246,16 @@ constexpr int kZScanCols = 4;  // columns (wavefronts) per 256-thread workgroup
 
 // unknowns per lane (power of two), 0 when the column is too long for the
 // register path (n > 64 kZScanMaxCh)
+// Columns per wavefront of the ensemble's register z-solve: short columns
+// (at most 2 unknowns per lane, 128 levels) are solved four to a wavefront,
+// so that a wavefront has more than one or two 512-byte rows of loads in
+// flight (48 members of 64x48x64 .. 192x128x128: 712 -> 555 us per
+// component).  The single-run z-solve keeps one column per wavefront.
+#ifndef SLC_ZSCAN_SHORT_K
+#define SLC_ZSCAN_SHORT_K 4
+#endif
+__host__ __device__ constexpr int zscan_k(int ch) { return ch <= 2 ? SLC_ZSCAN_SHORT_K : 1; }
+
 __host__ __device__ inline int zsolve_scan_ch(int comp, int nz) {
     const int n = comp == 2 ? nz - 1 : nz;
     int ch = 1;
@@ -481,25 +491,18 @@ __device__ inline void zsolve_scan_cols(const Grid& g, double* __restrict__ F,
     }
 }
 
-template <int C, int CH>
-__device__ inline void zsolve_scan_col(const Grid& g, double* __restrict__ F,
-                                       const double* __restrict__ tab, long col,
-                                       double beta = 0.0,
-                                       const double* __restrict__ dz_c = nullptr,
-                                       const double* __restrict__ dz_f = nullptr) {
-    zsolve_scan_cols<C, CH, 1>(g, F, tab, col, col + 1, beta, dz_c, dz_f);
-}
-
-// Runtime CH dispatch of zsolve_scan_col (ch from zsolve_scan_ch), theta = 1.
+// Runtime CH dispatch (ch from zsolve_scan_ch), theta = 1: the zscan_k(ch)
+// columns col0 .. of one wavefront (those below ncols).
 template <int C>
 __device__ inline void zsolve_scan_any(const Grid& g, double* __restrict__ F,
-                                       const double* __restrict__ tab, long col, int ch) {
+                                       const double* __restrict__ tab, long col0, long ncols,
+                                       int ch) {
     switch (ch) {
-        case 1: zsolve_scan_col<C, 1>(g, F, tab, col); break;
-        case 2: zsolve_scan_col<C, 2>(g, F, tab, col); break;
-        case 4: zsolve_scan_col<C, 4>(g, F, tab, col); break;
-        case 8: zsolve_scan_col<C, 8>(g, F, tab, col); break;
-        default: zsolve_scan_col<C, 16>(g, F, tab, col); break;
+        case 1: zsolve_scan_cols<C, 1, zscan_k(1)>(g, F, tab, col0, ncols); break;
+        case 2: zsolve_scan_cols<C, 2, zscan_k(2)>(g, F, tab, col0, ncols); break;
+        case 4: zsolve_scan_cols<C, 4, zscan_k(4)>(g, F, tab, col0, ncols); break;
+        case 8: zsolve_scan_cols<C, 8, zscan_k(8)>(g, F, tab, col0, ncols); break;
+        default: zsolve_scan_cols<C, 16, zscan_k(16)>(g, F, tab, col0, ncols); break;
     }
 }
 
