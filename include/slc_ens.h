@@ -91,11 +91,18 @@ int slc_ens_step(slc_ens* ens, void* stream);
  *                        after membership changes), 0: launch eagerly
  * SLC_ENS_OPT_BDF2_FUSED 1 (default): one SL tile pass per component (both feet
  *                        and gathers of the BDF2 members), 0: far pass, then
- *                        near pass (as SLC_OPT_BDF2_FUSED; same doubles) */
+ *                        near pass (as SLC_OPT_BDF2_FUSED; same doubles)
+ * SLC_ENS_OPT_POOL       released member arenas kept for reuse by a later
+ *                        member of the same geometry (default 8, >= 0); beyond
+ *                        it slc_ens_remove frees the oldest, and hipFree waits
+ *                        for the device.  slc_ens_bytes counts the pool;
+ *                        slc_ens_trim_pool frees it on demand.  (No counterpart
+ *                        in the reference, which has no ensemble.) */
 #define SLC_ENS_OPT_TILE 1
 #define SLC_ENS_OPT_GRID 2
 #define SLC_ENS_OPT_GRAPH 3
 #define SLC_ENS_OPT_BDF2_FUSED 4
+#define SLC_ENS_OPT_POOL 5
 int slc_ens_set_option(slc_ens* ens, int option, int value);
 int slc_ens_get_option(const slc_ens* ens, int option, int* value);
 

@@ -890,6 +890,16 @@ int slc_ens_set_option(slc_ens* e, int option, int value) {
                 e->dirty = true;  // the captured step holds the other passes
             }
             return SLC_OK;
+        case SLC_ENS_OPT_POOL: {
+            if (value < 0) return fail(SLC_EINVAL, "pool size %d must be >= 0", value);
+            DeviceGuard dg(e->device);
+            e->pool_max = (size_t)value;
+            while (e->pool.size() > e->pool_max) {
+                free_member(e->pool.front());
+                e->pool.erase(e->pool.begin());
+            }
+            return SLC_OK;
+        }
         default:
             return fail(SLC_EINVAL, "unknown ensemble option %d", option);
     }
@@ -903,6 +913,7 @@ int slc_ens_get_option(const slc_ens* e, int option, int* value) {
         case SLC_ENS_OPT_GRID: *value = e->grid_per_cu; return SLC_OK;
         case SLC_ENS_OPT_GRAPH: *value = e->use_graph ? 1 : 0; return SLC_OK;
         case SLC_ENS_OPT_BDF2_FUSED: *value = e->bdf2_fused; return SLC_OK;
+        case SLC_ENS_OPT_POOL: *value = (int)e->pool_max; return SLC_OK;
         default: return fail(SLC_EINVAL, "unknown ensemble option %d", option);
     }
 }

@@ -21,7 +21,7 @@ from . import _lib
 from .grid import generate_grid
 from .plan import field_shapes
 
-OPT_TILE, OPT_GRID, OPT_GRAPH, OPT_BDF2_FUSED = 1, 2, 3, 4
+OPT_TILE, OPT_GRID, OPT_GRAPH, OPT_BDF2_FUSED, OPT_POOL = 1, 2, 3, 4, 5
 N_STATS = 5  # u_bulk, forcing, clamped feet, steps, u_bulk before the shift
 
 

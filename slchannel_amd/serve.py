@@ -51,7 +51,8 @@ import numpy as np
 import torch
 
 from .diagnostics import BlowupGuard
-from .ensemble import N_STATS, Ensemble, _INIT_CACHE, _config_grid, initial_fields, member_bytes
+from .ensemble import (N_STATS, OPT_POOL, Ensemble, _INIT_CACHE, _config_grid, initial_fields,
+                       member_bytes)
 
 
 @dataclass
@@ -236,6 +237,10 @@ class Server:
                  batch_points=1 << 22, small_points=None):
         self.ens = Ensemble(order=order, storage=storage, device=device, graph=graph)
         self.device = self.ens.device
+        # released member arenas stay pooled for reuse up to the member cap:
+        # the budget check counts them and trims the oldest when a request
+        # needs their room, so a retirement never waits on hipFree
+        self.ens.set_option(OPT_POOL, int(max_members))
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info(self.device)
             hbm_budget = int(0.6 * free)
