@@ -262,15 +262,32 @@ __device__ inline void make_st(const Grid& g, double x, double y, double z, Foot
 // Stages the stencil bounding box `b` of a tile's feet (field S) into `lds`,
 // using at most cap_bytes: the box (z fastest), then one int per (x, y) row,
 // its global base.  Returns false (nothing staged) when the box does not fit.
-// Ends with a barrier.
+// b.kmin becomes the z index of the box's first level.  Ends with a barrier.
+//
+// Double fields whose z rows have an even length and whose base is 16-byte
+// aligned are staged with gfx950's direct global-to-LDS loads of 16-byte
+// pairs (global_load_lds_dwordx4: no register round trip, no LDS write
+// instruction): the box then starts at an even level and has an even
+// z extent, so every pair lies inside one row (the rounded-up end is at most
+// the row length, which is even) and is aligned.  The barrier's release
+// fence waits for the loads.  Other fields (the w component at even nz, FP32
+// storage) are staged through registers.
+#ifndef SLC_STAGE_DMA
+#define SLC_STAGE_DMA 1
+#endif
 template <typename T, int C, int P>
-__device__ inline bool stage_box(const Grid& g, const T* __restrict__ S, const SBox& b,
+__device__ inline bool stage_box(const Grid& g, const T* __restrict__ S, SBox& b,
                                  T* __restrict__ lds, long cap_bytes, int& BY, int& BZ) {
+    bool dma = false;
+    if constexpr (SLC_STAGE_DMA && sizeof(T) == 8)
+        dma = (Comp<C>::SZ(g) % 2 == 0) && ((reinterpret_cast<uintptr_t>(S) & 15) == 0);
+    const int k0 = dma ? (b.kmin & ~1) : b.kmin;
     // extents in 64 bits, each bounded before the product: feet far outside
     // the domain (or non-finite ones, whose floors saturate) would overflow
     // int extents and must take the direct-load path
-    const long ex = (long)b.xmax - b.xmin + P, ey = (long)b.ymax - b.ymin + P,
-               ez = (long)b.kmax - b.kmin + P;
+    const long ex = (long)b.xmax - b.xmin + P, ey = (long)b.ymax - b.ymin + P;
+    long ez = (long)b.kmax - k0 + P;
+    if (dma) ez += ez & 1;
     const bool fits = b.xmin <= b.xmax && ex <= cap_bytes && ey <= cap_bytes &&
                       ez <= cap_bytes &&
                       ex * ey * ez * (long)sizeof(T) + ex * ey * 4 <= cap_bytes;
@@ -280,33 +297,61 @@ __device__ inline bool stage_box(const Grid& g, const T* __restrict__ S, const S
     const int rows = BX * BY;
     const int nbox = rows * BZ;
     if (fits) {
+        b.kmin = k0;
         int* rowbase = reinterpret_cast<int*>(lds + nbox);
         for (int r = threadIdx.x; r < rows; r += kSB) {
             const int x = r / BY, y = r - (r / BY) * BY;
             rowbase[r] = node_addr<C>(g, wrap_idx(b.xmin + x, g.nx),
-                                      wrap_idx(b.ymin + y, g.ny), b.kmin);
+                                      wrap_idx(b.ymin + y, g.ny), k0);
         }
         __syncthreads();
-        // flat element loop (z fastest: coalesced rows), e = row BZ + zz
-        // carried from one iteration to the next: e grows by kSB = dq BZ + dr
-        // with dr < BZ, so zz + dr < 2 BZ needs at most one carry
-        const int dq = kSB / BZ, dr = kSB - dq * BZ;
-        int row = (int)threadIdx.x / BZ;
-        int zz = (int)threadIdx.x - row * BZ;
-        // unrolled: the iterations are independent, so several row-base
-        // reads and global loads are in flight before the first wait
-        // (measured at 256^3 order 6: fused pass 2.05 -> 1.92 ms)
+        if (dma) {
+            // pair q = 2 elements; a wavefront's 64 lanes load 64 consecutive
+            // pairs into 1 KB of consecutive LDS per instruction; (row, zz) of
+            // element 2q carried from one iteration to the next as below
+            const int npairs = nbox / 2;
+            const int lane = (int)threadIdx.x & 63;
+            const int step = 2 * kSB;  // elements per iteration
+            const int dq = step / BZ, dr = step - dq * BZ;
+            int q = (int)threadIdx.x;
+            int row = 2 * q / BZ;
+            int zz = 2 * q - row * BZ;
+#pragma unroll 2
+            for (int base = q - lane; base < npairs; base += kSB) {
+                if (q < npairs)
+                    __builtin_amdgcn_global_load_lds(
+                        (const __attribute__((address_space(1))) void*)(S + rowbase[row] + zz),
+                        (__attribute__((address_space(3))) void*)(lds + 2 * base), 16, 0, 0);
+                q += kSB;
+                row += dq;
+                zz += dr;
+                if (zz >= BZ) {
+                    zz -= BZ;
+                    ++row;
+                }
+            }
+        } else {
+            // flat element loop (z fastest: coalesced rows), e = row BZ + zz
+            // carried from one iteration to the next: e grows by kSB = dq BZ + dr
+            // with dr < BZ, so zz + dr < 2 BZ needs at most one carry
+            const int dq = kSB / BZ, dr = kSB - dq * BZ;
+            int row = (int)threadIdx.x / BZ;
+            int zz = (int)threadIdx.x - row * BZ;
+            // unrolled: the iterations are independent, so several row-base
+            // reads and global loads are in flight before the first wait
+            // (measured at 256^3 order 6: fused pass 2.05 -> 1.92 ms)
 #ifndef SLC_STAGE_UNROLL
 #define SLC_STAGE_UNROLL 4
 #endif
 #pragma unroll SLC_STAGE_UNROLL
-        for (int e = threadIdx.x; e < nbox; e += kSB) {
-            lds[e] = S[rowbase[row] + zz];
-            row += dq;
-            zz += dr;
-            if (zz >= BZ) {
-                zz -= BZ;
-                ++row;
+            for (int e = threadIdx.x; e < nbox; e += kSB) {
+                lds[e] = S[rowbase[row] + zz];
+                row += dq;
+                zz += dr;
+                if (zz >= BZ) {
+                    zz -= BZ;
+                    ++row;
+                }
             }
         }
     }
@@ -518,7 +563,7 @@ __device__ __forceinline__ void sl_tile_body(
     const T* __restrict__ S = (MODE == MODE_FAR || MODE == MODE_BDF2) ? Sm : Sn;
     // MODE_BDF2: the near box is reduced with the far one and waits in
     // sh[6..11], so no lane holds it through the far contraction
-    const SBox b = (MODE == MODE_BDF2) ? s_block_box2(l, ln, sh) : s_block_box(l, sh);
+    SBox b = (MODE == MODE_BDF2) ? s_block_box2(l, ln, sh) : s_block_box(l, sh);
     int BY = 0, BZ = 0;
     const bool fits = stage_box<T, C, P>(g, S, b, lds, LDS_BYTES, BY, BZ);
     if (MODE == MODE_GATHER && defer) {  // fits is uniform over the block
@@ -540,7 +585,7 @@ __device__ __forceinline__ void sl_tile_body(
         // the near stencils' weights, before the near box is staged
         if (HOIST && idx[0] >= 0) zweights<C, P>(g, nst[threadIdx.x], w0);
     }
-    const SBox bn = (MODE == MODE_BDF2) ? SBox{sh[6], sh[7], sh[8], sh[9], sh[10], sh[11]} : b;
+    SBox bn = (MODE == MODE_BDF2) ? SBox{sh[6], sh[7], sh[8], sh[9], sh[10], sh[11]} : b;
     const bool fits_n = (MODE == MODE_BDF2) ? stage_box<T, C, P>(g, Sn, bn, lds, LDS_BYTES, BY, BZ)
                                             : fits;
 #pragma unroll
