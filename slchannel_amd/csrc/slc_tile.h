@@ -281,16 +281,24 @@ __device__ inline bool stage_box(const Grid& g, const T* __restrict__ S, SBox& b
     bool dma = false;
     if constexpr (SLC_STAGE_DMA && sizeof(T) == 8)
         dma = (Comp<C>::SZ(g) % 2 == 0) && ((reinterpret_cast<uintptr_t>(S) & 15) == 0);
-    const int k0 = dma ? (b.kmin & ~1) : b.kmin;
     // extents in 64 bits, each bounded before the product: feet far outside
     // the domain (or non-finite ones, whose floors saturate) would overflow
     // int extents and must take the direct-load path
     const long ex = (long)b.xmax - b.xmin + P, ey = (long)b.ymax - b.ymin + P;
+    auto fits_z = [&](long ez) {
+        return b.xmin <= b.xmax && ex <= cap_bytes && ey <= cap_bytes && ez <= cap_bytes &&
+               ex * ey * ez * (long)sizeof(T) + ex * ey * 4 <= cap_bytes;
+    };
+    // the even start and extent can add two levels: a box that fits only
+    // without them is staged through registers
+    if (dma) {
+        const long ezd = (long)b.kmax - (b.kmin & ~1) + P;
+        dma = fits_z(ezd + (ezd & 1));
+    }
+    const int k0 = dma ? (b.kmin & ~1) : b.kmin;
     long ez = (long)b.kmax - k0 + P;
     if (dma) ez += ez & 1;
-    const bool fits = b.xmin <= b.xmax && ex <= cap_bytes && ey <= cap_bytes &&
-                      ez <= cap_bytes &&
-                      ex * ey * ez * (long)sizeof(T) + ex * ey * 4 <= cap_bytes;
+    const bool fits = fits_z(ez);
     const int BX = fits ? (int)ex : 0;
     BY = fits ? (int)ey : 1;
     BZ = fits ? (int)ez : 1;
