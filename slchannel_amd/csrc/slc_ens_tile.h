@@ -18,7 +18,7 @@ __global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
 ens_tile_kernel(const EnsMember* __restrict__ mem,
                                                        const unsigned* __restrict__ start,
                                                        const int* __restrict__ umap) {
-    __shared__ T lds[kEnsLds / sizeof(T)];
+    __shared__ __align__(16) T lds[kEnsLds / sizeof(T)];  // 16-byte pairs land here (stage_box)
     __shared__ int sh[PASS == 2 ? 12 : 6];
     __shared__ int s_cnt;
     __shared__ FootSt nst[PASS == 2 ? TI * TJ * TK : 1];

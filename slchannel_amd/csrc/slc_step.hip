@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
     const double* __restrict__ Fm, const T* __restrict__ Sn, const T* __restrict__ Sm,
     double dt, double* __restrict__ out, Feet ft, unsigned long long* nclamp,
     unsigned* defer) {
-    __shared__ T lds[LDS_BYTES / sizeof(T)];
+    __shared__ __align__(16) T lds[LDS_BYTES / sizeof(T)];  // 16-byte pairs land here (stage_box)
     __shared__ int sh[MODE == MODE_BDF2 ? 12 : 6];
     __shared__ int s_cnt;
     __shared__ FootSt nst[MODE == MODE_BDF2 ? TI * TJ * TK : 1];  // near stencils (MODE_BDF2)
@@ -125,7 +125,7 @@ template <int C, int P>
 __global__ __launch_bounds__(kSB) __attribute__((amdgpu_waves_per_eu(SLC_WIDE_WAVES))) void gather_wide_kernel(
     Grid g, ArrivalTabs A, const double* __restrict__ F, Feet ft, double* __restrict__ out,
     unsigned long long* nclamp, const unsigned* __restrict__ flags, long wtiles) {
-    __shared__ double lds[kWideLds / sizeof(double)];
+    __shared__ __align__(16) double lds[kWideLds / sizeof(double)];
     __shared__ int sh[6];
     __shared__ int s_cnt;
     __shared__ unsigned long long s_want[kSB / 64];
