@@ -324,7 +324,10 @@ __device__ inline bool stage_box(const Grid& g, const T* __restrict__ S, SBox& b
             int q = (int)threadIdx.x;
             int row = 2 * q / BZ;
             int zz = 2 * q - row * BZ;
-#pragma unroll 2
+#ifndef SLC_STAGE_DMA_UNROLL
+#define SLC_STAGE_DMA_UNROLL 2  // 1, 2, 4 measured the same at 256^3
+#endif
+#pragma unroll SLC_STAGE_DMA_UNROLL
             for (int base = q - lane; base < npairs; base += kSB) {
                 if (q < npairs)
                     __builtin_amdgcn_global_load_lds(
