@@ -407,6 +407,17 @@ def test_tiled_gather_matches_direct(check, order):
             check(f"{comp} shape {shape} bit-identical", torch.equal(out, ref) and
                   int(ncl.item()) == int(ncl0.item()),
                   f"max|d|={(out - ref).abs().max().item():.1e} ncl={int(ncl.item())}")
+        # the field 8 bytes past a 16-byte boundary: boxes staged through
+        # registers instead of the direct-to-LDS pairs, same doubles
+        buf = torch.empty(f.numel() + 1, dtype=torch.float64, device="cuda")
+        fo = buf[1:].view(f.shape)
+        fo.copy_(f)
+        plan.set_option(plan.OPT_GATHER_TILE, 0)
+        out = torch.empty(n, dtype=torch.float64, device="cuda")
+        plan.gather(comp, fo, xd, yd, zd, out)
+        check(f"{comp} field at an 8-byte offset: bit-identical",
+              fo.data_ptr() % 16 == 8 and torch.equal(out, ref),
+              f"ptr%16={fo.data_ptr() % 16} max|d|={(out - ref).abs().max().item():.1e}")
 
 
 @pytest.mark.parametrize("order", [4, 6])
