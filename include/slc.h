@@ -236,6 +236,20 @@ int slc_init_solenoidal(slc_plan* plan, int W, int L, const double* ab, const do
                         double* u, double* v, double* w, void* stream);
 int64_t slc_init_solenoidal_work(int nx, int ny, int nz, int W);
 
+/* Trilinear interpolation of a source field src (sx x sy x sz doubles, z
+ * contiguous) onto nxq x nyq x nzq tensor-product query nodes, given per axis
+ * as the lower source index and the weight of the upper node (ix[a] with
+ * ix[a] + 1 < sx, tx[a] in [0, 1]; likewise y and z; device arrays):
+ * out[a][b][c] = sum_{p,q,r} ((wx_p wy_q) wz_r) src[ix+p][iy+q][iz+r],
+ * wx_0 = 1 - tx, wx_1 = tx, accumulated from zero in (p, q, r) order.  An
+ * index out of range gives NaN at that node.  Replaces _trilinear
+ * (initflow.py:31-44) of initialize_flow_interpolated (initflow.py:47-120,
+ * the restart onto another grid).  One launch on the stream, on device
+ * `device`. */
+int slc_interp_trilinear(int device, const double* src, int sx, int sy, int sz, const int* ix,
+                         const double* tx, int nxq, const int* iy, const double* ty, int nyq,
+                         const int* iz, const double* tz, int nzq, double* out, void* stream);
+
 /* Exact bulk-flux constraint: u += U_bulk - ub(u) on the interior, ghosts of
  * u refreshed.  out (4 device doubles): [0] correction, [1] forcing =
  * correction/dt, [2] bulk velocity after the shift, [3] before it.

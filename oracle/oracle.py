@@ -691,6 +691,69 @@ def run_diagnostics(u, v, w, p, nx, ny, nz, dx, dy, dz_f, dz_c, z_c, nu, dt,
 # ---------------------------------------------------------------------------
 
 
+def interp1d_weights(coords, nodes):
+    """initflow.py:20-28: lower index and upper weight of each coordinate in
+    the sorted nodes (searchsorted left, clamped into [1, len-1]; weights
+    clamped to [0, 1]: constant extrapolation outside)."""
+    k = np.clip(np.searchsorted(nodes, coords, side="left"), 1, len(nodes) - 1)
+    k0 = k - 1
+    denom = np.maximum(nodes[k] - nodes[k0], 1e-300)
+    t = np.clip((coords - nodes[k0]) / denom, 0.0, 1.0)
+    return k0, t
+
+
+def trilinear_tensor(field, xn, yn, zn, xq, yq, zq):
+    """initflow.py:31-44: trilinear interpolation of field (len xn, yn, zn)
+    onto the tensor-product query nodes, accumulated in (a, b, c) order."""
+    ix, tx = interp1d_weights(xq, xn)
+    iy, ty = interp1d_weights(yq, yn)
+    iz, tz = interp1d_weights(zq, zn)
+    out = np.zeros((len(xq), len(yq), len(zq)))
+    for a in (0, 1):
+        wx = (tx if a else 1.0 - tx).reshape(-1, 1, 1)
+        for b in (0, 1):
+            wy = (ty if b else 1.0 - ty).reshape(1, -1, 1)
+            for c in (0, 1):
+                wz = (tz if c else 1.0 - tz).reshape(1, 1, -1)
+                out += wx * wy * wz * field[np.ix_(ix + a, iy + b, iz + c)]
+    return out
+
+
+def interpolate_fields(u_s, v_s, w_s, z_c_s, z_f_s, Lx_s, Ly_s, nx, ny, nz, Lx, Ly, z_c, z_f):
+    """initflow.py:71-117 (initialize_flow_interpolated after loading): the
+    source's periodic ghosts refreshed, x/y mapped proportionally onto the
+    source box, z directly; returns u, v, w on the target staggered grid."""
+    u_s, v_s, w_s = u_s.copy(), v_s.copy(), w_s.copy()
+    nxs, nys = u_s.shape[0] - 1, v_s.shape[1] - 1
+    dxs, dys = Lx_s / nxs, Ly_s / nys
+    u_s[0, :, :] = u_s[-1, :, :]
+    u_s[:, 0, :] = u_s[:, -2, :]
+    u_s[:, -1, :] = u_s[:, 1, :]
+    v_s[0, :, :] = v_s[-2, :, :]
+    v_s[-1, :, :] = v_s[1, :, :]
+    v_s[:, 0, :] = v_s[:, -1, :]
+    w_s[0, :, :] = w_s[-2, :, :]
+    w_s[-1, :, :] = w_s[1, :, :]
+    w_s[:, 0, :] = w_s[:, -2, :]
+    w_s[:, -1, :] = w_s[:, 1, :]
+    x_face_s = np.arange(nxs + 1, dtype=np.float64) * dxs
+    x_cent_s = (np.arange(nxs + 2, dtype=np.float64) - 0.5) * dxs
+    y_face_s = np.arange(nys + 1, dtype=np.float64) * dys
+    y_cent_s = (np.arange(nys + 2, dtype=np.float64) - 0.5) * dys
+    dx, dy = Lx / nx, Ly / ny
+    rx, ry = Lx_s / Lx, Ly_s / Ly
+    x_face_t = np.arange(nx + 1, dtype=np.float64) * dx * rx
+    x_cent_t = (np.arange(nx + 2, dtype=np.float64) - 0.5) * dx * rx
+    y_face_t = np.arange(ny + 1, dtype=np.float64) * dy * ry
+    y_cent_t = (np.arange(ny + 2, dtype=np.float64) - 0.5) * dy * ry
+    x_cent_t = np.clip(x_cent_t, x_cent_s[0], x_cent_s[-1])
+    y_cent_t = np.clip(y_cent_t, y_cent_s[0], y_cent_s[-1])
+    u = trilinear_tensor(u_s, x_face_s, y_cent_s, z_c_s, x_face_t, y_cent_t, z_c)
+    v = trilinear_tensor(v_s, x_cent_s, y_face_s, z_c_s, x_cent_t, y_face_t, z_c)
+    w = trilinear_tensor(w_s, x_cent_s, y_cent_s, z_f_s, x_cent_t, y_cent_t, z_f)
+    return u, v, w
+
+
 class ChannelOracle:
     """Restates SLChannelFlow's SL-BDF2 step (solver.py:694-1018) on numpy
     arrays.  Fields are ghost-shaped float64 arrays set by the caller."""

@@ -21,7 +21,7 @@ EXPORTS = (
     "slc_zsolve", "slc_divergence", "slc_poisson", "slc_project", "slc_project_step",
     "slc_pressure_combine",
     "slc_bulk_velocity", "slc_bulk_force", "slc_bulk_force_noalloc", "slc_diagnostics",
-    "slc_init_solenoidal", "slc_init_solenoidal_work",
+    "slc_init_solenoidal", "slc_init_solenoidal_work", "slc_interp_trilinear",
     "slc_imex_rhs",
     "slc_ab2_update", "slc_set_option", "slc_set_viscosity",
     "slc_profile_enable", "slc_profile_report",
@@ -91,6 +91,7 @@ def load():
         "slc_diagnostics": ([vp, vp, vp, vp, vp, d, vp, vp], i),
         "slc_init_solenoidal": ([vp, i, i, vp, vp, vp, d, vp, ctypes.c_int64, vp, vp, vp, vp], i),
         "slc_init_solenoidal_work": ([i, i, i, i], ctypes.c_int64),
+        "slc_interp_trilinear": ([i, vp, i, i, i, vp, vp, i, vp, vp, i, vp, vp, i, vp, vp], i),
         "slc_imex_rhs": ([vp, vp, vp, vp, vp, vp, vp, vp], i),
         "slc_ab2_update": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, d, vp], i),
         "slc_set_option": ([vp, i, i], i),

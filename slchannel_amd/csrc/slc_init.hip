@@ -229,3 +229,64 @@ extern "C" int slc_init_solenoidal(slc_plan* p, int W, int L, const double* ab,
     HIP_OK(hipGetLastError());
     return SLC_OK;
 }
+
+// ---------------------------------------------------------------------------
+// trilinear interpolation onto tensor-product nodes (restart onto another grid)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// out[a][b][c] = sum over p, q, r in {0, 1} of ((wx_p wy_q) wz_r) src[ix+p][iy+q][iz+r]
+// with wx_0 = 1 - tx, wx_1 = tx (likewise y, z), accumulated from zero in
+// (p, q, r) order: the operations of initflow.py:31-44.  An index whose
+// upper node is outside the source writes NaN.
+__global__ __launch_bounds__(kIB) void interp_tri_kernel(
+    const double* __restrict__ src, int sx, int sy, int sz, const int* __restrict__ ix,
+    const double* __restrict__ tx, int nxq, const int* __restrict__ iy,
+    const double* __restrict__ ty, int nyq, const int* __restrict__ iz,
+    const double* __restrict__ tz, int nzq, double* __restrict__ out) {
+    const long N = (long)nxq * nyq * nzq;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const int c = (int)(t % nzq);
+    const long r = t / nzq;
+    const int b = (int)(r % nyq), a = (int)(r / nyq);
+    const int i0 = ix[a], j0 = iy[b], k0 = iz[c];
+    if (i0 < 0 || i0 + 1 >= sx || j0 < 0 || j0 + 1 >= sy || k0 < 0 || k0 + 1 >= sz) {
+        out[t] = __builtin_nan("");
+        return;
+    }
+    const double wx[2] = {1.0 - tx[a], tx[a]};
+    const double wy[2] = {1.0 - ty[b], ty[b]};
+    const double wz[2] = {1.0 - tz[c], tz[c]};
+    double acc = 0.0;
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int rr = 0; rr < 2; ++rr)
+                acc = acc + ((wx[p] * wy[q]) * wz[rr]) *
+                                src[((long)(i0 + p) * sy + (j0 + q)) * sz + (k0 + rr)];
+    out[t] = acc;
+}
+
+}  // namespace
+
+extern "C" int slc_interp_trilinear(int device, const double* src, int sx, int sy, int sz,
+                                    const int* ix, const double* tx, int nxq, const int* iy,
+                                    const double* ty, int nyq, const int* iz, const double* tz,
+                                    int nzq, double* out, void* stream) {
+    if (!src || !ix || !tx || !iy || !ty || !iz || !tz || !out)
+        return fail(SLC_EINVAL, "null array");
+    if (sx < 2 || sy < 2 || sz < 2 || nxq < 1 || nyq < 1 || nzq < 1)
+        return fail(SLC_EINVAL, "source %dx%dx%d (each >= 2), query %dx%dx%d (each >= 1)", sx, sy,
+                    sz, nxq, nyq, nzq);
+    DeviceGuard dg(device);
+    const long N = (long)nxq * nyq * nzq;
+    hipLaunchKernelGGL(interp_tri_kernel, dim3((unsigned)((N + kIB - 1) / kIB)), dim3(kIB), 0,
+                       static_cast<hipStream_t>(stream), src, sx, sy, sz, ix, tx, nxq, iy, ty, nyq,
+                       iz, tz, nzq, out);
+    HIP_OK(hipGetLastError());
+    return SLC_OK;
+}

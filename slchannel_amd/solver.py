@@ -24,7 +24,7 @@ import torch
 import yaml
 
 from . import env
-from .checkpoint import initialize_flow_from_file, save_flow_fields
+from .checkpoint import initialize_flow_from_file, interpolated_fields, save_flow_fields
 from .grid import generate_grid
 from .initflow import solenoidal_fields
 from .plan import HipPlan, field_shapes
@@ -137,10 +137,12 @@ class SLChannelFlow:
 
         init = cfg.get("initialization") or {}
         field_file = init.get("field_file")
-        if field_file is not None:
-            if init.get("type", "parabolic") == "interpolate":
-                raise ValueError("initialization.type 'interpolate' (a field from another "
-                                 "grid) is not supported by the HIP layer")
+        if init.get("type", "parabolic") == "interpolate":
+            if field_file is None:
+                raise ValueError(
+                    "initialization.type 'interpolate' requires initialization.field_file")
+            self.initialize_interpolated(field_file, source_half=init.get("source_half", "lower"))
+        elif field_file is not None:
             self.restart_from(field_file, reset_time=bool(init.get("reset_time", False)))
 
     # ------------------------------------------------------------------
@@ -195,6 +197,27 @@ class SLChannelFlow:
         self.set_fields(u, v, w)
         self.apply_bc_uvw()
         self.project_initial()
+
+    def initialize_interpolated(self, field_file, source_half="lower"):
+        """Restart onto this grid from a stored field of another grid or box
+        (solver.py:351-401, initflow.py:47-120): trilinear interpolation of
+        each component at its staggered nodes on the device, boundary fill, u
+        rescaled to U_bulk (when the interpolated bulk velocity is not ~0),
+        then the initial projection (solver.py:599-618).  Step and time start
+        at zero; the next step is a BDF1 bootstrap."""
+        u, v, w, p = interpolated_fields(field_file, self.nx, self.ny, self.nz, self.Lx, self.Ly,
+                                         self.Lz, self.z_c, self.z_f, self.device,
+                                         source_half=source_half)
+        self.set_fields(u, v, w, p)
+        self.apply_bc_uvw()
+        self.plan.bulk_velocity(self.u, self._bulk)
+        ub = float(self._bulk[3].item())
+        if abs(ub) > 1e-9:
+            self.u *= self.U_bulk / ub
+        self.project_initial()
+        self.forcing = 0.0
+        self.initial_step = self.current_step = 0
+        self.initial_time = self.time = 0.0
 
     def project_initial(self):
         """solver.py:599-618: one projection with dt = 1 and a ghost fill."""
