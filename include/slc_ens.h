@@ -112,7 +112,8 @@ int slc_ens_get_option(const slc_ens* ens, int option, int* value);
  * fastest of each.  Leaves every member's state unchanged (the timed
  * kernels only write the predictor scratch, U*, the feet and spectral
  * buffers; clamp counters are cleared).  report (optional): "shape ms\n"
- * and "grid<k> ms\n" lines. */
+ * and "grid<k> ms\n" lines, each time printed exactly (%.9g of the float
+ * that was compared; equal times keep the first candidate). */
 int slc_ens_autotune(slc_ens* ens, int reps, void* stream, char* report, int64_t len);
 
 #ifdef __cplusplus

@@ -942,7 +942,7 @@ static int autotune_sweeps(slc_ens* e, int reps, hipEvent_t t0, hipEvent_t t1,
         (void)hipEventElapsedTime(&ms, t0, t1);
         ms /= (float)reps;
         char line[64];
-        snprintf(line, sizeof line, "%d %.4f\n", shape, ms);
+        snprintf(line, sizeof line, "%d %.9g\n", shape, ms);
         rep += line;
         if (ms < best_ms) {
             best_ms = ms;
@@ -981,7 +981,7 @@ static int autotune_sweeps(slc_ens* e, int reps, hipEvent_t t0, hipEvent_t t1,
         (void)hipEventElapsedTime(&ms, t0, t1);
         ms /= (float)reps;
         char line[64];
-        snprintf(line, sizeof line, "grid%d %.4f\n", grids[gi], ms);
+        snprintf(line, sizeof line, "grid%d %.9g\n", grids[gi], ms);
         rep += line;
         if (ms < best_gms) {
             best_gms = ms;
