@@ -899,18 +899,39 @@ __device__ inline void bulk_partial_body(const Grid& g, double* __restrict__ u,
     const long beg = blk * chunk;
     const long end = beg + chunk < N ? beg + chunk : N;
     double acc = 0.0;
-    for (long n = beg + threadIdx.x; n < end; n += kRedBlock) {
+    long n = beg + threadIdx.x;
+    if (n < end) {
+        // (i, j, k) of the first element, then steps of kRedBlock elements in
+        // the mixed radix (ny, nz): dk < nz and dj < ny, so one carry per
+        // digit suffices.  Same elements in the same order as a division per
+        // element.
         const unsigned un = (unsigned)n;
         int k = (int)(un % (unsigned)g.nz);
-        unsigned r = un / (unsigned)g.nz;
+        const unsigned r = un / (unsigned)g.nz;
         int j = (int)(r % (unsigned)g.ny), i = (int)(r / (unsigned)g.ny);
-        long ui = ((long)(i + 1) * (g.ny + 2) + (j + 1)) * (g.nz + 2) + (k + 1);
-        double val = u[ui];
-        if (shift) {
-            val = val + corr;
-            u[ui] = val;
+        const int dk = kRedBlock % g.nz, t = kRedBlock / g.nz;
+        const int dj = t % g.ny, di = t / g.ny;
+#pragma unroll 4
+        for (; n < end; n += kRedBlock) {
+            const long ui = ((long)(i + 1) * (g.ny + 2) + (j + 1)) * (g.nz + 2) + (k + 1);
+            double val = u[ui];
+            if (shift) {
+                val = val + corr;
+                u[ui] = val;
+            }
+            acc += val * cv[k];
+            k += dk;
+            if (k >= g.nz) {
+                k -= g.nz;
+                ++j;
+            }
+            j += dj;
+            if (j >= g.ny) {
+                j -= g.ny;
+                ++i;
+            }
+            i += di;
         }
-        acc += val * cv[k];
     }
     double s = block_sum(acc, red);
     if (threadIdx.x == 0) partials[blk] = s;
