@@ -74,12 +74,12 @@ def parse():
                          "step at the reference's Eulerian CFL 0.28, with both schemes' "
                          "simulated time per second")
     ap.add_argument("--members", type=int, default=48, help="config 4: ensemble size")
-    ap.add_argument("--ensemble-grids", default="survey", choices=("survey", "small"),
+    ap.add_argument("--ensemble-grids", default="workload", choices=("workload", "small"),
                     help="config 4: member grids -- the workload's four sizes, or small "
                          "launch-bound members (32x24x32, 48x32x48, 32x32x24)")
     ap.add_argument("--requests", type=int, default=24, help="config 5: number of requests")
     ap.add_argument("--rate", type=float, default=1.0, help="config 5: arrivals per second")
-    ap.add_argument("--serve-grids", default="survey", choices=("survey", "small"),
+    ap.add_argument("--serve-grids", default="workload", choices=("workload", "small"),
                     help="config 5: request grids -- the workload's four sizes, or small "
                          "launch-bound ones (32x24x32, 48x32x48, 32x32x24)")
     ap.add_argument("--hbm-budget-gb", type=float, default=48.0,
@@ -511,7 +511,7 @@ def ensemble_bench(args):
     from slchannel_amd.diagnostics import cfl_dt
     from slchannel_amd.ensemble import OPT_GRAPH, Ensemble
 
-    grids = ENSEMBLE_GRIDS if args.ensemble_grids == "survey" else SMALL_GRIDS
+    grids = ENSEMBLE_GRIDS if args.ensemble_grids == "workload" else SMALL_GRIDS
     rng = np.random.Generator(np.random.PCG64(4000))
     specs = []
     for m in range(args.members):

@@ -12,7 +12,7 @@ order-4/6 tensor-product Lagrange gather, all in float64.
 (slchannel/semilag_triton.py:347-455): ``departure(comp, dt) -> n`` and
 ``gather(comp, buf, n) -> Tensor``, reading the trajectory velocity set by the
 last ``advect`` (or ``set_mid``) call.  It is what a maintainer binds in place
-of ``TritonSL`` (see INTEGRATION.md).
+of ``TritonSL``.
 
 There is no CPU or eager fallback: constructing either class without a ROCm
 device or without the built library raises.
