@@ -90,7 +90,7 @@ def parse():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build runs the "
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this library runs the "
               "channel on a single MI355X and has no multi-GPU path.", file=sys.stderr)
         sys.exit(2)
     import torch

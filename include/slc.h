@@ -217,7 +217,7 @@ int slc_diagnostics(slc_plan* plan, const double* u, const double* v, const doub
 /* Seeded solenoidal initial velocity on the device (no counterpart in the
  * reference, whose initialisers are the parabolic / vortex / uniform fields of
  * initflow.py:123-223; the seeded perturbation of SLChannelFlow.initialize is
- * this build's own).  Writes the interiors of u, v, w (ghost-shaped; ghosts
+ * this library's own).  Writes the interiors of u, v, w (ghost-shaped; ghosts
  * untouched): the curl of A_c = sum_b Re[a^c_b e^{i(kx x + ky y)}] G_l(z)
  * over W x W wavenumber bins (W odd, bins (m, n) with |m|, |n| <= (W-1)/2)
  * and l = 1..L, scaled to the given u'rms over the three interiors, plus
