@@ -244,3 +244,19 @@ def test_solver_interpolate_start(check, tmp_path):
     flow.step_sl_bdf2(0.01)
     check("first step runs (BDF1 bootstrap) and stays finite",
           bool(torch.isfinite(flow.u).all()) and not torch.equal(flow.u, u0), "")
+
+
+def test_taller_source_needs_lower_half(check, tmp_path):
+    """initflow.py:78-79: only the lower part of a taller source is supported;
+    another source_half is refused before any device work."""
+    from slchannel_amd.checkpoint import interpolated_fields
+
+    path, _ = _write_source(tmp_path, 6, 4, 8, LX, LY, 2.0, 9)
+    z_f, z_c = _zgrid(4, 1.0)
+    try:
+        interpolated_fields(path, 6, 4, 4, LX, LY, 1.0, z_c, z_f, "cpu", source_half="upper")
+        msg = None
+    except ValueError as e:
+        msg = str(e)
+    check("source_half 'upper' with a taller source raises ValueError",
+          msg is not None and "source_half" in msg, f"{msg}")
